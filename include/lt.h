@@ -1,0 +1,385 @@
+/*
+ * lt.h — C ABI of the MI355X light-transport core.
+ *
+ * Every entry point returns an lt_status; no C++ exception crosses this
+ * boundary.  All structures are plain data with the byte layouts of the
+ * reference's host/device wire format (src/shaders/commons.h) so that a
+ * LumenScene's CPU vectors can be handed over unchanged.
+ *
+ * Threading: calls on one lt_context are not thread-safe; each context owns one
+ * HIP stream.  lt_render / lt_trace_* with device pointers are asynchronous on
+ * that stream; every entry point that returns host data synchronises it.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef LT_H
+#define LT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LT_ABI_VERSION 1
+
+typedef enum lt_status {
+    LT_OK = 0,
+    LT_ERR_INVALID_ARG = 1,   /* null pointer, bad size, out-of-range index   */
+    LT_ERR_HIP = 2,           /* a HIP runtime call failed                    */
+    LT_ERR_OUT_OF_MEMORY = 3, /* device or host allocation failed             */
+    LT_ERR_NOT_READY = 4,     /* e.g. render before scene upload / accel build */
+    LT_ERR_PARSE = 5,         /* scene JSON / OBJ parse error                 */
+    LT_ERR_UNSUPPORTED = 6,   /* feature not built                            */
+    LT_ERR_NO_DEVICE = 7      /* no HIP device visible                        */
+} lt_status;
+
+/* ---- reference wire format (src/shaders/commons.h) ------------------------ */
+
+/* BSDF type flags, commons.h:13-18 */
+#define LT_BSDF_DIFFUSE 1u
+#define LT_BSDF_MIRROR 2u
+#define LT_BSDF_GLASS 4u
+#define LT_BSDF_DIELECTRIC 8u
+#define LT_BSDF_CONDUCTOR 16u
+#define LT_BSDF_PRINCIPLED 32u
+/* BSDF property flags, commons.h:22-26 */
+#define LT_BSDF_FLAG_DIFFUSE 1u
+#define LT_BSDF_FLAG_SPECULAR 2u
+#define LT_BSDF_FLAG_GLOSSY 4u
+#define LT_BSDF_FLAG_REFLECTION 8u
+#define LT_BSDF_FLAG_TRANSMISSION 16u
+/* Light type = flags & 7, commons.h:36-38; bit 4 finite, bit 5 delta
+ * (commons.glsl:42-46).  LT_LIGHT_POINT is an addition of this core: an omni
+ * point light (finite + delta), which the reference data model lacks. */
+#define LT_LIGHT_SPOT 1u
+#define LT_LIGHT_AREA 2u
+#define LT_LIGHT_DIRECTIONAL 3u
+#define LT_LIGHT_POINT 4u
+#define LT_LIGHT_FINITE (1u << 4)
+#define LT_LIGHT_DELTA (1u << 5)
+
+/* Vertex, commons.h:194-198 (32 B) */
+typedef struct lt_vertex {
+    float pos[3];
+    float normal[3];
+    float uv0[2];
+} lt_vertex;
+
+/* PrimMeshInfo, commons.h:333-340 (48 B) */
+typedef struct lt_prim_mesh_info {
+    uint32_t index_offset;
+    uint32_t vertex_offset;
+    uint32_t material_index;
+    uint32_t pad;
+    float min_pos[4];
+    float max_pos[4];
+} lt_prim_mesh_info;
+
+/* Material, commons.h:212-234 (104 B).  texture_id indexes lt_scene_desc
+ * textures and multiplies albedo (bsdf_commons.glsl:16-22).  A roughness
+ * texture per material, which the reference lacks, is given by
+ * lt_scene_desc.roughness_texture_ids. */
+typedef struct lt_material {
+    float albedo[3];
+    float ior;
+    float emissive_factor[3];
+    uint32_t bsdf_type;
+    uint32_t bsdf_props;
+    float k[3];
+    int32_t texture_id;
+    float roughness;
+    float diffuse_trans;
+    float spec_trans;
+    float metallic;
+    float specular_tint;
+    float sheen_tint;
+    float clearcoat;
+    float clearcoat_gloss;
+    float sheen;
+    float subsurface;
+    float flatness;
+    float anisotropy;
+    uint32_t thin;
+} lt_material;
+
+/* Light, commons.h:200-210 (128 B).  world_matrix is column-major (glm). */
+typedef struct lt_light {
+    float world_matrix[16];
+    float pos[3];
+    uint32_t prim_mesh_idx;
+    float to[3];
+    uint32_t num_triangles;
+    float L[3];
+    uint32_t light_flags;
+    float world_center[3];
+    float world_radius;
+} lt_light;
+
+/* A float RGBA texture (texel (x,y) at data[4*(y*width+x)]), sampled
+ * bilinearly with repeat addressing (the reference sampler,
+ * src/RayTracer/LumenScene.cpp:190-195).  Texels are linear floats. */
+typedef struct lt_texture {
+    uint32_t width;
+    uint32_t height;
+    const float* data;
+} lt_texture;
+
+/* Flattened scene: the CPU vectors of LumenScene (src/RayTracer/LumenScene.h:47-99)
+ * after load_scene (src/RayTracer/LumenScene.cpp:52-229).  Copied at upload; the
+ * caller keeps ownership.  Vertices of meshes whose world matrix is not the
+ * identity must already be in world space (the reference's JSON path always
+ * uses identity world matrices, LumenScene.cpp:328). */
+typedef struct lt_scene_desc {
+    const lt_vertex* vertices;
+    uint32_t num_vertices;
+    const uint32_t* indices;
+    uint32_t num_indices;
+    const lt_prim_mesh_info* prim_infos;
+    uint32_t num_prim_infos;
+    const lt_material* materials;
+    uint32_t num_materials;
+    const lt_light* lights;
+    uint32_t num_lights;
+    const lt_texture* textures;
+    uint32_t num_textures;
+    /* per material: texture whose .x replaces roughness, or -1; NULL = none */
+    const int32_t* roughness_texture_ids;
+    /* per light: spot cone (cos_width, cos_falloff_start); NULL = the reference's
+     * fixed 30 / 25 degrees (commons.glsl:264-265).  An addition of this core. */
+    const float* spot_cones;
+    uint32_t total_light_triangle_cnt; /* LumenScene.cpp:91,106 */
+    float total_light_area;            /* LumenScene.cpp:111-129 */
+    int32_t dir_light_idx;             /* -1 if none, LumenScene.cpp:107-109 */
+    float scene_min[3];                /* m_dimensions, LumenScene.cpp:732-751 */
+    float scene_max[3];
+} lt_scene_desc;
+
+/* SceneUBO subset used by the kernels (commons.h:180-192): column-major
+ * matrices as produced by Integrator::update_uniform_buffers
+ * (src/RayTracer/Integrator.cpp:60-72). */
+typedef struct lt_camera {
+    float view[16];
+    float projection[16];
+    float inv_view[16];
+    float inv_projection[16];
+} lt_camera;
+
+/* A ray for the parity / microbenchmark entry points (32 B). */
+typedef struct lt_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} lt_ray;
+
+/* Closest-hit result (16 B).  prim = BLAS-local triangle index
+ * (gl_PrimitiveID), inst = mesh index (gl_InstanceCustomIndexEXT); both are
+ * 0xFFFFFFFF on a miss.  u, v are the barycentric weights of v1 and v2. */
+typedef struct lt_hit {
+    float t;
+    uint32_t prim;
+    uint32_t inst;
+    float u;
+    float v;
+} lt_hit;
+
+/* ---- context -------------------------------------------------------------- */
+
+typedef struct lt_context lt_context;
+typedef struct lt_integrator lt_integrator;
+
+/* Replaces vk::init (src/Framework/VulkanBase.cpp:619): selects HIP device
+ * `device`, creates one stream. */
+lt_status lt_context_create(int device, lt_context** out);
+/* Replaces Integrator::destroy / LumenScene::destroy
+ * (src/RayTracer/Integrator.cpp:128-135, LumenScene.cpp:753-766).  Idempotent
+ * on NULL. */
+lt_status lt_context_destroy(lt_context* ctx);
+/* Human-readable text of the last error on this context (never NULL). */
+const char* lt_last_error(const lt_context* ctx);
+/* Number of compute units and the gfx arch name of the context's device. */
+lt_status lt_device_info(lt_context* ctx, int* num_cus, char* arch, size_t arch_len);
+
+/* Replaces LumenScene::load_scene's buffer uploads (LumenScene.cpp:146-216). */
+lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* scene);
+
+/* Build options for lt_accel_build. */
+typedef struct lt_build_opts {
+    uint32_t max_leaf_tris; /* triangles per BVH8 leaf child, 1..4 (default 3) */
+    uint32_t reserved[7];
+} lt_build_opts;
+
+/* Build statistics. */
+typedef struct lt_build_stats {
+    uint32_t num_nodes;     /* BVH8 nodes (128 B each)          */
+    uint32_t num_tris;      /* triangle records (48 B each)     */
+    uint32_t max_depth;     /* BVH8 depth (root = 1)            */
+    uint32_t binary_depth;  /* depth of the LBVH binary tree    */
+    float build_ms;         /* device time of the whole build   */
+} lt_build_stats;
+
+/* Replaces Integrator::create_accel -> vk::build_blas / vk::build_tlas
+ * (src/RayTracer/Integrator.cpp:137-160, src/Framework/AccelerationStructure.cpp:171-315):
+ * LBVH (30-bit Morton, radix sort, Karras radix tree) collapsed to a BVH8 of
+ * 128-byte quantised nodes.  opts may be NULL. */
+lt_status lt_accel_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats);
+/* Copies the built BVH8 (128-B nodes) and its 48-B triangle records to host
+ * memory (for inspection and for host traversals over the identical tree).
+ * No reference counterpart: the reference's BVH is opaque to the host. */
+lt_status lt_accel_export(lt_context* ctx, void* nodes, size_t nodes_bytes, void* tris, size_t tris_bytes);
+
+/* ---- raw queries (parity and microbenchmarks) ---------------------------- */
+
+#define LT_PTR_DEVICE 1u /* rays/hits are device pointers; call is asynchronous */
+
+/* Closest hit: the traceRayEXT(tlas, gl_RayFlagsOpaqueEXT, ...) query plus the
+ * ray.rchit / ray.rmiss identity fields (src/shaders/ray.rchit:24-84,
+ * ray.rmiss:13).  Ties in t resolve to the smaller (inst, prim). */
+lt_status lt_trace_closest(lt_context* ctx, const lt_ray* rays, size_t n, lt_hit* hits, uint32_t flags);
+/* Occlusion: traceRayEXT(TerminateOnFirstHit|SkipClosestHit) + ray.rahit /
+ * ray_shadow.rmiss (src/shaders/integrators/pt_commons.glsl:20-23,
+ * ray.rahit:18-20, ray_shadow.rmiss:12).  occluded[i] = 1 iff any triangle is
+ * hit with tmin <= t <= tmax. */
+lt_status lt_trace_any(lt_context* ctx, const lt_ray* rays, size_t n, uint8_t* occluded, uint32_t flags);
+/* Per-ray traversal work of the last lt_trace_closest / lt_trace_any call in a
+ * counting pass: total BVH8 nodes visited and triangles tested. */
+lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_hit, uint32_t flags,
+                         uint64_t* nodes, uint64_t* tris);
+
+/* Device-time of the last kernel launch issued by lt_trace_* / lt_render, in ms
+ * (HIP events on the context stream). */
+lt_status lt_last_kernel_ms(lt_context* ctx, float* ms);
+
+/* Device buffers (for callers that keep rays/hits resident in HBM). */
+lt_status lt_device_alloc(lt_context* ctx, size_t bytes, void** dptr);
+lt_status lt_device_free(lt_context* ctx, void* dptr);
+lt_status lt_memcpy_h2d(lt_context* ctx, void* dst, const void* src, size_t bytes);
+lt_status lt_memcpy_d2h(lt_context* ctx, void* dst, const void* src, size_t bytes);
+lt_status lt_synchronize(lt_context* ctx);
+
+/* ---- integrators ------------------------------------------------------------ */
+
+/* IntegratorType names of src/RayTracer/SceneConfig.h:13 that this core builds. */
+typedef enum lt_integrator_kind {
+    LT_INTEGRATOR_PATH = 0,
+    LT_INTEGRATOR_BDPT = 1,
+    LT_INTEGRATOR_VCM = 2
+} lt_integrator_kind;
+
+/* Path execution strategies. */
+#define LT_PATH_MEGAKERNEL 0u /* one thread per pixel, whole path in one kernel */
+#define LT_PATH_WAVEFRONT 1u  /* persistent stages with compacted ray queues   */
+
+/* Path integrator config: SceneConfig/PathConfig (src/RayTracer/SceneConfig.h:15-25)
+ * plus the PCPath fields set by Path::render (src/RayTracer/Path.cpp:27-38). */
+typedef struct lt_path_config {
+    uint32_t width;
+    uint32_t height;
+    int32_t max_depth;        /* path_length (default 6)                 */
+    float sky_col[3];
+    uint32_t direct_lighting; /* Path.h: default 1                       */
+    uint32_t strategy;        /* LT_PATH_MEGAKERNEL or LT_PATH_WAVEFRONT  */
+    uint32_t reserved[6];
+} lt_path_config;
+
+/* Per-frame parameters (SceneUBO + push-constant frame_num). */
+typedef struct lt_frame_params {
+    lt_camera camera;
+    uint32_t frame_num; /* running-mean index: 0 overwrites */
+    uint32_t seed;      /* XOR'ed into the RNG frame word (0 = reference Path) */
+    uint32_t spp;       /* frames rendered by this call (>=1); frame_num advances */
+    uint32_t reserved[5];
+} lt_frame_params;
+
+/* Render statistics of the last lt_render call. */
+typedef struct lt_render_stats {
+    uint64_t closest_rays; /* closest-hit queries traced */
+    uint64_t shadow_rays;  /* occlusion queries traced   */
+    float kernel_ms;       /* device time of the render  */
+    uint32_t frames;
+} lt_render_stats;
+
+/* Replaces RayTracer::create_integrator + Integrator::init
+ * (src/RayTracer/RayTracer.cpp:244-282, Integrator.cpp:8-50).  cfg points to
+ * the config struct of `kind` (lt_path_config for LT_INTEGRATOR_PATH) and
+ * cfg_size must equal its size. */
+lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const void* cfg, size_t cfg_size,
+                               lt_integrator** out);
+lt_status lt_integrator_destroy(lt_integrator* it);
+/* Replaces Integrator::render + RenderGraph::run (src/RayTracer/Path.cpp:27-59):
+ * renders params->spp frames starting at params->frame_num into the running
+ * mean.  Asynchronous. */
+lt_status lt_render(lt_integrator* it, const lt_frame_params* params);
+/* Ray counts and device time of the last lt_render (synchronises; with
+ * counting enabled the kernels tally rays). */
+lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* stats);
+/* Enables per-ray counting in later lt_render calls (slower; for Mrays/s). */
+lt_status lt_render_set_counting(lt_integrator* it, int enable);
+/* Reset of the accumulation (Path::update with updated=true, Path.cpp:61-68). */
+lt_status lt_accum_reset(lt_integrator* it);
+/* Copies output_tex (RGBA32F, row-major: texel (x,y) at 4*(y*width+x)) to host. */
+lt_status lt_readback_rgba32f(lt_integrator* it, float* host, size_t n_floats);
+
+/* ---- metrics ------------------------------------------------------------------- */
+
+/* Squared-error sum between two device or host RGBA32F images over rgb:
+ * S = sum_px |a.rgb - b.rgb|^2.  The reference's RMSE is sqrt(S)/(3*W*H)
+ * (src/shaders/rmse/output_rmse.comp:23); standard RMSE is sqrt(S/(3*W*H)). */
+lt_status lt_image_sq_error(lt_context* ctx, const float* a, const float* b, size_t n_pixels, uint32_t flags,
+                            double* sum_sq);
+
+/* ---- scenes ---------------------------------------------------------------------- */
+
+/* A host-side scene owned by the library: the flattened LumenScene vectors and
+ * the camera of its JSON, as produced by a loader or a procedural generator. */
+typedef struct lt_host_scene lt_host_scene;
+
+/* Procedural workloads. */
+typedef enum lt_procedural_kind {
+    LT_SCENE_CORNELL = 0,   /* Cornell box: 5 wall quads, 2 rotated boxes, 0.25-area light, 36 triangles */
+    LT_SCENE_TERRAIN = 1,   /* fBm heightfield + instanced boxes, Disney materials, point + area lights  */
+    LT_SCENE_CAUSTICS = 2,  /* glass sphere on a diffuse plane, one spot light                           */
+    LT_SCENE_DIFF_WALL = 3  /* Cornell box with a tessellated, textured back wall                        */
+} lt_procedural_kind;
+
+/* Procedural generator parameters; zero fields take the documented default. */
+typedef struct lt_procedural_params {
+    uint32_t seed;
+    uint32_t terrain_tris;   /* TERRAIN: heightfield triangles (default 1,000,000) */
+    uint32_t num_instances;  /* TERRAIN: instanced boxes (default 2,000)           */
+    uint32_t num_point_lights; /* TERRAIN: point lights (default 64)              */
+    uint32_t sphere_tris;    /* CAUSTICS: sphere triangles (default 20,000)        */
+    uint32_t wall_tris;      /* DIFF_WALL: back-wall triangles (default 16,384)    */
+    uint32_t tex_res;        /* DIFF_WALL: texture resolution (default 256)        */
+    uint32_t reserved[9];
+} lt_procedural_params;
+
+lt_status lt_scene_procedural(lt_procedural_kind kind, const lt_procedural_params* params, lt_host_scene** out);
+/* Lumen JSON + OBJ loader (src/RayTracer/LumenScene.cpp:231-513). */
+lt_status lt_scene_load_json(const char* path, lt_host_scene** out, char* err, size_t err_len);
+lt_status lt_scene_free(lt_host_scene* s);
+/* Borrowed view of the scene arrays (valid until lt_scene_free). */
+lt_status lt_scene_view(const lt_host_scene* s, lt_scene_desc* out);
+/* Camera of the scene for a given image size: PerspectiveCamera(fov, 0.01, 1000,
+ * W/H, dir, pos) then update_view_matrix (src/Framework/Camera.h:28-38,67-85,
+ * 109-121; src/RayTracer/Integrator.cpp:60-72). */
+lt_status lt_scene_camera(const lt_host_scene* s, uint32_t width, uint32_t height, lt_camera* out);
+/* Integrator block of the scene JSON (LumenScene.cpp:237-269): type
+ * (lt_integrator_kind), path_length, sky_col, VCM enable_vm / radius_factor. */
+typedef struct lt_scene_integrator_cfg {
+    int32_t kind;
+    int32_t path_length;
+    float sky_col[3];
+    int32_t enable_vm;
+    float radius_factor;
+} lt_scene_integrator_cfg;
+lt_status lt_scene_integrator(const lt_host_scene* s, lt_scene_integrator_cfg* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LT_H */

@@ -1,0 +1,82 @@
+"""Ahead-of-time build of the HIP core into lumen_amd/liblt.so (gfx950).
+
+Every translation unit under lumen_amd/csrc is compiled by hipcc in parallel
+and linked into one shared library; the C ABI is include/lt.h.  Run as
+``python -m lumen_amd.build`` or call :func:`build`.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OBJ = PKG / "_build"
+LIB = PKG / "liblt.so"
+
+ARCH = os.environ.get("LT_OFFLOAD_ARCH", "gfx950")
+
+COMMON = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    f"-I{ROOT / 'include'}",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+    "-Wno-unused-result",
+]
+# Packed FP32 instructions are disabled for all device code.
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build lumen_amd)")
+
+
+def _compile(src: Path) -> Path:
+    out = OBJ / (src.name + ".o")
+    cmd = [_hipcc(), *COMMON]
+    if src.suffix == ".hip":
+        cmd += ["-x", "hip", *DEVICE]
+    else:
+        # host-only C++ that calls the HIP runtime API
+        cmd += ["-D__HIP_PLATFORM_AMD__"]
+    cmd += ["-c", str(src), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {src.name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(verbose: bool = False) -> Path:
+    """Compiles every source and links lumen_amd/liblt.so; returns its path."""
+    OBJ.mkdir(exist_ok=True)
+    srcs = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp")])
+    jobs = min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,512 @@
+// bvh_build.hip — GPU LBVH build collapsed to a quantised BVH8 (gfx950).
+//
+// Replaces the driver build behind Integrator::create_accel ->
+// vk::build_blas / vk::build_tlas (src/RayTracer/Integrator.cpp:137-160,
+// src/Framework/AccelerationStructure.cpp:171-315).  Stages:
+//   1. k_make_tris     : 48-B triangle records + centroid bounds (atomics)
+//   2. k_morton        : 30-bit Morton codes of the centroids
+//   3. radix sort      : (morton, triangle) pairs, hipcub/rocPRIM onesweep
+//   4. k_karras        : binary radix tree (Karras 2012), one thread per node
+//   5. k_refit         : bottom-up AABBs with arrival counters
+//   6. k_collapse      : top-down BVH8 collapse, one thread per BVH8 node and
+//                        level; greedy expansion of the child with the largest
+//                        surface area, subtrees of <= max_leaf triangles become
+//                        leaves; 16-bit quantised, 128-B nodes.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+#include <cmath>
+#include <vector>
+
+#include "context.h"
+
+using ltb::Node8;
+using ltb::TriRec;
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// one thread per triangle: find the mesh by binary search over index offsets
+__global__ void k_make_tris(const lt_vertex* __restrict__ vtx, const uint32_t* __restrict__ idx,
+                            const lt_prim_mesh_info* __restrict__ prims, uint32_t nprims, uint32_t ntri,
+                            TriRec* __restrict__ out, uint32_t* __restrict__ cbounds /* 6 ordered uints */) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (g < ntri) {
+        uint32_t lo = 0, hi = nprims;  // last mesh with index_offset <= 3g
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (prims[mid].index_offset <= 3 * g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const lt_prim_mesh_info pi = prims[lo];
+        const uint32_t io = 3 * g;
+        TriRec r;
+        const float* a = vtx[idx[io] + pi.vertex_offset].pos;
+        const float* b = vtx[idx[io + 1] + pi.vertex_offset].pos;
+        const float* c = vtx[idx[io + 2] + pi.vertex_offset].pos;
+        for (int k = 0; k < 3; k++) {
+            r.v0[k] = a[k];
+            r.v1[k] = b[k];
+            r.v2[k] = c[k];
+            float cen = (a[k] + b[k] + c[k]) * (1.0f / 3.0f);
+            clo[k] = cen;
+            chi[k] = cen;
+        }
+        r.gid = g;
+        r.prim = (io - pi.index_offset) / 3;
+        r.inst = lo;
+        out[g] = r;
+    }
+    // block reduction of the centroid bounds, then one atomic per block
+    for (int k = 0; k < 3; k++) {
+        float l = clo[k], h = chi[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            l = fminf(l, __shfl_xor(l, o));
+            h = fmaxf(h, __shfl_xor(h, o));
+        }
+        if ((threadIdx.x & 63) == 0 && l <= h) {
+            atomicMin(&cbounds[k], f2ord(l));
+            atomicMax(&cbounds[3 + k], f2ord(h));
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const TriRec* __restrict__ tris, uint32_t n, const uint32_t* __restrict__ cb,
+                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const TriRec r = tris[g];
+    uint32_t code = 0;
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float lo = ord2f(cb[k]), hi = ord2f(cb[3 + k]);
+        float cen = (r.v0[k] + r.v1[k] + r.v2[k]) * (1.0f / 3.0f);
+        float ext = hi - lo;
+        float t = ext > 0.0f ? (cen - lo) / ext : 0.5f;
+        q[k] = (uint32_t)fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+    }
+    code = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    keys[g] = code;
+    vals[g] = g;
+}
+
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz(a ^ b);
+}
+
+// Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees
+// and k-d trees".  Children: bit 31 set = leaf (index into sorted order).
+__global__ void k_karras(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ left,
+                         uint32_t* __restrict__ right, uint32_t* __restrict__ parent_int,
+                         uint32_t* __restrict__ parent_leaf, uint32_t* __restrict__ range_lo,
+                         uint32_t* __restrict__ range_hi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t lc = (lo == gamma) ? (0x80000000u | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t rc = (hi == gamma + 1) ? (0x80000000u | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    range_lo[i] = (uint32_t)lo;
+    range_hi[i] = (uint32_t)hi;
+    if (lc & 0x80000000u)
+        parent_leaf[lc & 0x7fffffffu] = (uint32_t)i;
+    else
+        parent_int[lc] = (uint32_t)i;
+    if (rc & 0x80000000u)
+        parent_leaf[rc & 0x7fffffffu] = (uint32_t)i;
+    else
+        parent_int[rc] = (uint32_t)i;
+}
+
+__device__ __forceinline__ Box tri_box(const TriRec& r) {
+    Box b;
+    for (int k = 0; k < 3; k++) {
+        b.lo[k] = fminf(r.v0[k], fminf(r.v1[k], r.v2[k]));
+        b.hi[k] = fmaxf(r.v0[k], fmaxf(r.v1[k], r.v2[k]));
+    }
+    return b;
+}
+
+// bottom-up refit: second arrival at a node computes its box
+__global__ void k_refit(const TriRec* __restrict__ tris, const uint32_t* __restrict__ sorted, int n,
+                        const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                        const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
+                        uint32_t* __restrict__ flags, Box* __restrict__ ibox, Box* __restrict__ lbox) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lbox[i] = tri_box(tris[sorted[i]]);
+    if (n == 1) return;
+    __threadfence();
+    uint32_t p = parent_leaf[i];
+    for (;;) {
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;
+        __threadfence();
+        uint32_t a = left[p], b = right[p];
+        // volatile reads: the sibling's box was written by another wave
+        const volatile Box* ba = (a & 0x80000000u) ? (const volatile Box*)&lbox[a & 0x7fffffffu]
+                                                   : (const volatile Box*)&ibox[a];
+        const volatile Box* bb = (b & 0x80000000u) ? (const volatile Box*)&lbox[b & 0x7fffffffu]
+                                                   : (const volatile Box*)&ibox[b];
+        Box r;
+        for (int k = 0; k < 3; k++) {
+            r.lo[k] = fminf(ba->lo[k], bb->lo[k]);
+            r.hi[k] = fmaxf(ba->hi[k], bb->hi[k]);
+        }
+        ibox[p] = r;
+        __threadfence();
+        if (p == 0) return;
+        p = parent_int[p];
+    }
+}
+
+struct Task {
+    uint32_t bnode;  // binary node (bit 31: leaf)
+    uint32_t slot;   // BVH8 node index
+};
+
+__device__ __forceinline__ float half_area(const Box& b) {
+    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ uint32_t count_of(uint32_t c, const uint32_t* rlo, const uint32_t* rhi) {
+    if (c & 0x80000000u) return 1;
+    return rhi[c] - rlo[c] + 1;
+}
+
+__device__ __forceinline__ Box box_of(uint32_t c, const Box* ibox, const Box* lbox) {
+    return (c & 0x80000000u) ? lbox[c & 0x7fffffffu] : ibox[c];
+}
+
+__global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __restrict__ out,
+                           uint32_t* __restrict__ ctr /* [0] nodes, [1] tris, [2] out tasks */,
+                           const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                           const uint32_t* __restrict__ rlo, const uint32_t* __restrict__ rhi,
+                           const Box* __restrict__ ibox, const Box* __restrict__ lbox,
+                           const uint32_t* __restrict__ sorted, const TriRec* __restrict__ tris_in,
+                           TriRec* __restrict__ tris_out, Node8* __restrict__ nodes, uint32_t max_leaf) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_in) return;
+    const Task task = in[g];
+    uint32_t ch[8];
+    int nc = 0;
+    if (task.bnode & 0x80000000u) {
+        ch[nc++] = task.bnode;  // a single-triangle scene
+    } else if (count_of(task.bnode, rlo, rhi) <= max_leaf) {
+        ch[nc++] = task.bnode;  // root small enough to be one leaf
+    } else {
+        ch[nc++] = left[task.bnode];
+        ch[nc++] = right[task.bnode];
+        while (nc < 8) {
+            int best = -1;
+            float best_a = -1.0f;
+            for (int c = 0; c < nc; c++) {
+                if (ch[c] & 0x80000000u) continue;
+                if (count_of(ch[c], rlo, rhi) <= max_leaf) continue;
+                float a = half_area(ibox[ch[c]]);
+                if (a > best_a) {
+                    best_a = a;
+                    best = c;
+                }
+            }
+            if (best < 0) break;
+            uint32_t b = ch[best];
+            ch[best] = left[b];
+            ch[nc++] = right[b];
+        }
+    }
+    // classify
+    uint32_t n_int = 0, n_leaf_tris = 0;
+    bool is_leaf[8];
+    for (int c = 0; c < nc; c++) {
+        uint32_t cnt = count_of(ch[c], rlo, rhi);
+        is_leaf[c] = (ch[c] & 0x80000000u) || cnt <= max_leaf;
+        if (is_leaf[c])
+            n_leaf_tris += cnt;
+        else
+            n_int++;
+    }
+    uint32_t child_base = n_int ? atomicAdd(&ctr[0], n_int) : 0u;
+    uint32_t tri_base = n_leaf_tris ? atomicAdd(&ctr[1], n_leaf_tris) : 0u;
+    uint32_t task_base = n_int ? atomicAdd(&ctr[2], n_int) : 0u;
+    // node box and quantisation
+    Box cb[8];
+    Box nb;
+    for (int k = 0; k < 3; k++) {
+        nb.lo[k] = INFINITY;
+        nb.hi[k] = -INFINITY;
+    }
+    for (int c = 0; c < nc; c++) {
+        cb[c] = box_of(ch[c], ibox, lbox);
+        for (int k = 0; k < 3; k++) {
+            nb.lo[k] = fminf(nb.lo[k], cb[c].lo[k]);
+            nb.hi[k] = fmaxf(nb.hi[k], cb[c].hi[k]);
+        }
+    }
+    Node8 nd;
+    int ex[3];
+    for (int k = 0; k < 3; k++) {
+        nd.p[k] = nb.lo[k];
+        double extent = (double)nb.hi[k] - (double)nb.lo[k];
+        double s = extent / 65535.0;
+        int e;
+        if (s <= 0.0) {
+            e = -100;
+        } else {
+            double m = frexp(s, &e);  // s = m * 2^e, m in [0.5, 1)
+            if (m == 0.5) e -= 1;
+            if (ldexp(65535.0, e) < extent) e += 1;
+            if (e < -100) e = -100;
+        }
+        ex[k] = e;
+        nd.e[k] = (int8_t)e;
+    }
+    nd.imask = 0;
+    nd.child_base = child_base;
+    nd.tri_base = tri_base;
+    uint32_t int_slot = 0, tri_off = 0;
+    for (int c = 0; c < 8; c++) {
+        if (c >= nc) {
+            nd.meta[c] = 0;
+            for (int k = 0; k < 3; k++) {
+                nd.qlo[k][c] = 0xFFFF;  // empty: lo > hi, never hit
+                nd.qhi[k][c] = 0;
+            }
+            continue;
+        }
+        for (int k = 0; k < 3; k++) {
+            double inv = ldexp(1.0, -ex[k]);
+            double ql = floor(((double)cb[c].lo[k] - (double)nd.p[k]) * inv);
+            double qh = ceil(((double)cb[c].hi[k] - (double)nd.p[k]) * inv);
+            nd.qlo[k][c] = (uint16_t)fmin(fmax(ql, 0.0), 65535.0);
+            nd.qhi[k][c] = (uint16_t)fmin(fmax(qh, 0.0), 65535.0);
+        }
+        if (is_leaf[c]) {
+            uint32_t first, cnt;
+            if (ch[c] & 0x80000000u) {
+                first = ch[c] & 0x7fffffffu;
+                cnt = 1;
+            } else {
+                first = rlo[ch[c]];
+                cnt = rhi[ch[c]] - rlo[ch[c]] + 1;
+            }
+            nd.meta[c] = (uint8_t)((tri_off << 3) | cnt);
+            for (uint32_t t = 0; t < cnt; t++) tris_out[tri_base + tri_off + t] = tris_in[sorted[first + t]];
+            tri_off += cnt;
+        } else {
+            nd.imask |= (uint8_t)(1u << c);
+            nd.meta[c] = (uint8_t)int_slot;
+            out[task_base + int_slot] = Task{ch[c], child_base + int_slot};
+            int_slot++;
+        }
+    }
+    nodes[task.slot] = nd;
+}
+
+}  // namespace
+
+lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats) {
+    if (!ctx->scene.uploaded) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_accel_build: no scene uploaded");
+    uint32_t max_leaf = opts && opts->max_leaf_tris ? opts->max_leaf_tris : 3u;
+    if (max_leaf < 1 || max_leaf > 4) return lt_fail(ctx, LT_ERR_INVALID_ARG, "max_leaf_tris must be in 1..4");
+    hipStream_t st = ctx->stream;
+    DeviceBVH& bvh = ctx->bvh;
+    if (bvh.nodes) (void)hipFree(bvh.nodes);
+    if (bvh.tris) (void)hipFree(bvh.tris);
+    bvh = DeviceBVH{};
+    const uint32_t n = ctx->scene.ni / 3;
+    LT_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    if (n == 0) {
+        bvh.built = true;
+        if (stats) *stats = lt_build_stats{};
+        return LT_OK;
+    }
+    // temporaries
+    TriRec* tris_unsorted = nullptr;
+    uint32_t *cb = nullptr, *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
+    uint32_t *left = nullptr, *right = nullptr, *pint = nullptr, *pleaf = nullptr, *rlo = nullptr, *rhi = nullptr,
+             *flags = nullptr, *ctr = nullptr;
+    Box *ibox = nullptr, *lbox = nullptr;
+    Task *q0 = nullptr, *q1 = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    lt_status rc = LT_OK;
+    auto fail = [&](hipError_t e, const char* what) {
+        rc = lt_hip_check(ctx, e, what);
+        return rc;
+    };
+#define BCHK(x)                                       \
+    do {                                              \
+        hipError_t _e = (x);                          \
+        if (_e != hipSuccess) {                       \
+            fail(_e, #x);                             \
+            goto cleanup;                             \
+        }                                             \
+    } while (0)
+    {
+        const uint32_t ni = n > 1 ? n - 1 : 1;
+        BCHK(hipMalloc(&tris_unsorted, sizeof(TriRec) * n));
+        BCHK(hipMalloc(&bvh.tris, sizeof(TriRec) * n));
+        BCHK(hipMalloc(&bvh.nodes, sizeof(Node8) * (size_t)std::max<uint32_t>(n, 1)));
+        BCHK(hipMalloc(&cb, 6 * sizeof(uint32_t)));
+        BCHK(hipMalloc(&keys, 4ull * n));
+        BCHK(hipMalloc(&vals, 4ull * n));
+        BCHK(hipMalloc(&keys2, 4ull * n));
+        BCHK(hipMalloc(&vals2, 4ull * n));
+        BCHK(hipMalloc(&left, 4ull * ni));
+        BCHK(hipMalloc(&right, 4ull * ni));
+        BCHK(hipMalloc(&pint, 4ull * ni));
+        BCHK(hipMalloc(&pleaf, 4ull * n));
+        BCHK(hipMalloc(&rlo, 4ull * ni));
+        BCHK(hipMalloc(&rhi, 4ull * ni));
+        BCHK(hipMalloc(&flags, 4ull * ni));
+        BCHK(hipMalloc(&ibox, sizeof(Box) * ni));
+        BCHK(hipMalloc(&lbox, sizeof(Box) * n));
+        BCHK(hipMalloc(&q0, sizeof(Task) * (size_t)n));
+        BCHK(hipMalloc(&q1, sizeof(Task) * (size_t)n));
+        BCHK(hipMalloc(&ctr, 4 * sizeof(uint32_t)));
+        uint32_t init_cb[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        BCHK(hipMemcpyAsync(cb, init_cb, sizeof(init_cb), hipMemcpyHostToDevice, st));
+        BCHK(hipMemsetAsync(flags, 0, 4ull * ni, st));
+        const int B = 256;
+        const uint32_t G = (n + B - 1) / B;
+        hipLaunchKernelGGL(k_make_tris, dim3(G), dim3(B), 0, st, ctx->scene.vtx, ctx->scene.idx, ctx->scene.prims,
+                           ctx->scene.nprims, n, tris_unsorted, cb);
+        BCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_morton, dim3(G), dim3(B), 0, st, tris_unsorted, n, cb, keys, vals);
+        BCHK(hipGetLastError());
+        BCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, st));
+        BCHK(hipMalloc(&sort_tmp, sort_bytes));
+        BCHK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, st));
+        if (n > 1) {
+            hipLaunchKernelGGL(k_karras, dim3((n - 1 + B - 1) / B), dim3(B), 0, st, keys2, (int)n, left, right, pint,
+                               pleaf, rlo, rhi);
+            BCHK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_refit, dim3(G), dim3(B), 0, st, tris_unsorted, vals2, (int)n, left, right, pint, pleaf,
+                           flags, ibox, lbox);
+        BCHK(hipGetLastError());
+        // top-down collapse, one launch per BVH8 level
+        uint32_t h_ctr[4] = {1u, 0u, 0u, 0u};  // node 0 is the root
+        BCHK(hipMemcpyAsync(ctr, h_ctr, sizeof(h_ctr), hipMemcpyHostToDevice, st));
+        Task root{n > 1 ? 0u : 0x80000000u, 0u};
+        BCHK(hipMemcpyAsync(q0, &root, sizeof(Task), hipMemcpyHostToDevice, st));
+        uint32_t n_in = 1, depth = 0;
+        while (n_in > 0) {
+            depth++;
+            BCHK(hipMemsetAsync(ctr + 2, 0, 4, st));
+            hipLaunchKernelGGL(k_collapse, dim3((n_in + 63) / 64), dim3(64), 0, st, q0, n_in, q1, ctr, left, right,
+                               rlo, rhi, ibox, lbox, vals2, tris_unsorted, bvh.tris, bvh.nodes, max_leaf);
+            BCHK(hipGetLastError());
+            BCHK(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
+            BCHK(hipStreamSynchronize(st));
+            n_in = h_ctr[2];
+            std::swap(q0, q1);
+        }
+        bvh.n_nodes = h_ctr[0];
+        bvh.n_tris = h_ctr[1];
+        bvh.depth = depth;
+        BCHK(hipEventRecord(ctx->ev1, st));
+        BCHK(hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        BCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (bvh.n_tris != n) {
+            rc = lt_fail(ctx, LT_ERR_HIP, "BVH collapse lost triangles");
+            goto cleanup;
+        }
+        if (7u * (depth - 1) + 1 > (uint32_t)ltb::kStackTotal) {
+            rc = lt_fail(ctx, LT_ERR_UNSUPPORTED,
+                         "BVH8 depth " + std::to_string(depth) + " exceeds the traversal stack bound");
+            goto cleanup;
+        }
+        {
+            // structural check before any traversal runs: every internal child
+            // index is larger than its parent's (so traversal terminates) and in
+            // range, and every leaf range lies inside the triangle array
+            std::vector<Node8> h(bvh.n_nodes);
+            BCHK(hipMemcpy(h.data(), bvh.nodes, sizeof(Node8) * h.size(), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < bvh.n_nodes; i++) {
+                const Node8& nd = h[i];
+                for (int c = 0; c < 8; c++) {
+                    const uint32_t meta = nd.meta[c];
+                    if ((nd.imask >> c) & 1u) {
+                        const uint64_t ch = (uint64_t)nd.child_base + meta;
+                        if (ch <= i || ch >= bvh.n_nodes) {
+                            rc = lt_fail(ctx, LT_ERR_HIP, "BVH8 node " + std::to_string(i) + " has a bad child");
+                            goto cleanup;
+                        }
+                    } else if (meta & 7u) {
+                        const uint64_t end = (uint64_t)nd.tri_base + (meta >> 3) + (meta & 7u);
+                        if (end > bvh.n_tris) {
+                            rc = lt_fail(ctx, LT_ERR_HIP, "BVH8 node " + std::to_string(i) + " has a bad leaf");
+                            goto cleanup;
+                        }
+                    }
+                }
+            }
+        }
+        bvh.built = true;
+        if (stats) {
+            stats->num_nodes = bvh.n_nodes;
+            stats->num_tris = bvh.n_tris;
+            stats->max_depth = depth;
+            stats->binary_depth = 0;
+            stats->build_ms = ms;
+        }
+    }
+cleanup:
+#undef BCHK
+    void* tmp[] = {tris_unsorted, cb, keys, vals, keys2, vals2, left, right, pint, pleaf, rlo, rhi, flags, ibox,
+                   lbox, q0, q1, ctr, sort_tmp};
+    for (void* p : tmp)
+        if (p) (void)hipFree(p);
+    if (rc != LT_OK) {
+        if (bvh.nodes) (void)hipFree(bvh.nodes);
+        if (bvh.tris) (void)hipFree(bvh.tris);
+        bvh = DeviceBVH{};
+    }
+    return rc;
+}

@@ -1,0 +1,102 @@
+// context.h — host-side state behind the C ABI (one HIP device, one stream).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/lt.h"
+#include "bvh_layout.h"
+
+// texture descriptor in HBM (texels are float4, row-major)
+struct TexDesc {
+    uint32_t width, height;
+    uint64_t offset;  // in float4 units into the texel pool
+};
+
+// Everything a kernel needs to know about the scene, passed by value.
+struct SceneView {
+    const lt_vertex* vtx;
+    const uint32_t* idx;
+    const lt_prim_mesh_info* prims;
+    const lt_material* mats;
+    const lt_light* lights;
+    const float* cones;      // 2 per light
+    const float4* texels;
+    const TexDesc* texs;
+    const int32_t* rough;    // per material
+    const ltb::Node8* nodes;
+    const ltb::TriRec* tris;
+    uint32_t num_lights;
+    uint32_t num_nodes;
+    uint32_t total_light_triangle_cnt;
+    int32_t dir_light_idx;
+    float total_light_area;
+    float scene_min[3], scene_max[3];
+};
+
+struct DeviceScene {
+    lt_vertex* vtx = nullptr;
+    uint32_t* idx = nullptr;
+    lt_prim_mesh_info* prims = nullptr;
+    lt_material* mats = nullptr;
+    lt_light* lights = nullptr;
+    float* cones = nullptr;
+    float4* texels = nullptr;
+    TexDesc* texs = nullptr;
+    int32_t* rough = nullptr;
+    uint32_t nv = 0, ni = 0, nprims = 0, nmats = 0, nlights = 0, ntex = 0;
+    uint32_t total_light_triangle_cnt = 0;
+    int32_t dir_light_idx = -1;
+    float total_light_area = 0.0f;
+    float smin[3] = {0, 0, 0}, smax[3] = {0, 0, 0};
+    bool uploaded = false;
+};
+
+struct DeviceBVH {
+    ltb::Node8* nodes = nullptr;
+    ltb::TriRec* tris = nullptr;
+    uint32_t n_nodes = 0, n_tris = 0, depth = 0;
+    bool built = false;
+};
+
+struct lt_context {
+    int device = 0;
+    int num_cus = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    DeviceScene scene;
+    DeviceBVH bvh;
+    // traversal stack spill (kStackTotal - kLdsStack entries per resident thread)
+    uint32_t* spill = nullptr;
+    size_t spill_threads = 0;
+    // persistent-kernel work counters
+    uint32_t* counters = nullptr;  // 64 words
+    // scratch for raw trace calls with host pointers
+    void* ray_buf = nullptr;
+    size_t ray_buf_bytes = 0;
+    void* hit_buf = nullptr;
+    size_t hit_buf_bytes = 0;
+    uint64_t* count_buf = nullptr;  // [nodes, tris]
+    float last_ms = 0.0f;
+};
+
+// helpers shared by the .cpp / .hip translation units
+lt_status lt_fail(lt_context* ctx, lt_status st, const std::string& msg);
+lt_status lt_hip_check(lt_context* ctx, hipError_t e, const char* what);
+#define LT_HIP(ctx, call)                                              \
+    do {                                                               \
+        hipError_t _e = (call);                                        \
+        if (_e != hipSuccess) return lt_hip_check((ctx), _e, #call);   \
+    } while (0)
+
+SceneView make_scene_view(const lt_context* ctx);
+// persistent grid: number of resident threads the spill buffer is sized for
+lt_status ensure_spill(lt_context* ctx, size_t threads);
+constexpr int kTraceBlock = 128;
+
+// builder / kernels (implemented in the .hip files)
+lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats);
+lt_status launch_trace(lt_context* ctx, const lt_ray* d_rays, size_t n, void* d_out, int any_hit, int count,
+                       uint64_t* d_counts);

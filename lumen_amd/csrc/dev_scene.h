@@ -1,0 +1,185 @@
+// dev_scene.h — closest-hit payload, material fetch and light sampling on
+// the device (src/shaders/ray.rchit, bsdf_commons.glsl:16-22,
+// commons.glsl:112-300).
+#pragma once
+#include "context.h"
+#include "dev_bsdf.h"
+#include "dev_math.h"
+
+namespace ltd {
+
+// HitPayload (utils.glsl:15-26); persists across traces like the GLSL
+// rayPayloadEXT: a miss only clears `valid` (material_idx = -1, ray.rmiss:13).
+struct Payload {
+    f3 n_g, n_s, pos;
+    f2 uv;
+    uint32_t material_idx, triangle_idx, instance_idx;
+    bool valid;
+};
+
+// ray.rchit:24-84 with identity object<->world transforms
+LT_DEV void fill_payload(const SceneView& sv, uint32_t inst, uint32_t prim, float u, float v, Payload& p) {
+    const lt_prim_mesh_info pi = sv.prims[inst];
+    const uint32_t io = pi.index_offset + 3 * prim;
+    const lt_vertex a = sv.vtx[sv.idx[io] + pi.vertex_offset];
+    const lt_vertex b = sv.vtx[sv.idx[io + 1] + pi.vertex_offset];
+    const lt_vertex c = sv.vtx[sv.idx[io + 2] + pi.vertex_offset];
+    const f3 v0 = ld3(a.pos), v1 = ld3(b.pos), v2 = ld3(c.pos);
+    const float b0 = 1.0f - u - v;
+    p.pos = v0 * b0 + v1 * u + v2 * v;
+    const f3 nrm = normalize(ld3(a.normal) * b0 + ld3(b.normal) * u + ld3(c.normal) * v);
+    p.n_s = normalize(nrm);
+    p.uv = {a.uv0[0] * b0 + b.uv0[0] * u + c.uv0[0] * v, a.uv0[1] * b0 + b.uv0[1] * u + c.uv0[1] * v};
+    p.n_g = normalize(cross(v2 - v0, v1 - v0));
+    p.material_idx = pi.material_index;
+    p.triangle_idx = prim;
+    p.instance_idx = inst;
+    p.valid = true;
+}
+
+// bilinear, repeat addressing, texel centres at (i + 0.5) / size
+LT_DEV float4 tex_bilinear(const SceneView& sv, int tid, f2 uv) {
+    const TexDesc t = sv.texs[tid];
+    float fx = uv.x * (float)t.width - 0.5f, fy = uv.y * (float)t.height - 0.5f;
+    float x0f = floorf(fx), y0f = floorf(fy);
+    float ax = fx - x0f, ay = fy - y0f;
+    int w = (int)t.width, h = (int)t.height;
+    int x0 = ((int)x0f % w + w) % w, y0 = ((int)y0f % h + h) % h;
+    int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
+    const float4* base = sv.texels + t.offset;
+    const float4 A = base[(size_t)y0 * w + x0], B = base[(size_t)y0 * w + x1];
+    const float4 C = base[(size_t)y1 * w + x0], D = base[(size_t)y1 * w + x1];
+    auto lerp2 = [&](float a, float b, float c, float d) {
+        float top = a * (1.0f - ax) + b * ax;
+        float bot = c * (1.0f - ax) + d * ax;
+        return top * (1.0f - ay) + bot * ay;
+    };
+    return make_float4(lerp2(A.x, B.x, C.x, D.x), lerp2(A.y, B.y, C.y, D.y), lerp2(A.z, B.z, C.z, D.z),
+                       lerp2(A.w, B.w, C.w, D.w));
+}
+
+// bsdf_commons.glsl:16-22 (+ per-material roughness texture)
+LT_DEV Mat load_material(const SceneView& sv, uint32_t idx, f2 uv) {
+    Mat m;
+    const uint4* src = (const uint4*)(sv.mats + idx);  // 104 B = 6.5 x 16 B
+    uint32_t* dst = (uint32_t*)&m;
+    const uint32_t* s32 = (const uint32_t*)(sv.mats + idx);
+#pragma unroll
+    for (int i = 0; i < 26; i++) dst[i] = s32[i];
+    (void)src;
+    if (m.texture_id > -1) {
+        float4 t = tex_bilinear(sv, m.texture_id, uv);
+        m.albedo = m.albedo * mk3(t.x, t.y, t.z);
+    }
+    if (sv.rough) {
+        int rt = sv.rough[idx];
+        if (rt > -1) m.roughness = tex_bilinear(sv, rt, uv).x;
+    }
+    return m;
+}
+
+struct LightRecord {
+    uint32_t light_idx, flags;
+    f2 bary;
+    uint32_t triangle_idx, instance_idx;
+};
+
+struct TriangleRecord {
+    f3 pos, n_s;
+    float triangle_pdf;
+};
+
+// commons.glsl:112-149 (identity world matrices)
+LT_DEV TriangleRecord sample_triangle(const SceneView& sv, const lt_prim_mesh_info& pi, float rx, float ry,
+                                      uint32_t tri, const float* world, f2& uv) {
+    TriangleRecord r;
+    const uint32_t io = pi.index_offset + 3 * tri;
+    const lt_vertex a = sv.vtx[sv.idx[io] + pi.vertex_offset];
+    const lt_vertex b = sv.vtx[sv.idx[io + 1] + pi.vertex_offset];
+    const lt_vertex c = sv.vtx[sv.idx[io + 2] + pi.vertex_offset];
+    const f3 v0 = ld3(a.pos), v1 = ld3(b.pos), v2 = ld3(c.pos);
+    const float sr = sqrtf(rx);
+    uv = {1.0f - sr, ry * sr};
+    const f3 bc = {1.0f - uv.x - uv.y, uv.x, uv.y};
+    const f3 d1 = v1 - v0, d2 = v2 - v0;
+    const f4 e0 = mul4(world, f4{d1.x, d1.y, d1.z, 1.0f});
+    const f4 e1 = mul4(world, f4{d2.x, d2.y, d2.z, 1.0f});
+    const f3 pos = v0 * bc.x + v1 * bc.y + v2 * bc.z;
+    const f3 nrm = normalize(ld3(a.normal) * bc.x + ld3(b.normal) * bc.y + ld3(c.normal) * bc.z);
+    const f4 wp = mul4(world, f4{pos.x, pos.y, pos.z, 1.0f});
+    r.n_s = normalize(nrm);
+    r.triangle_pdf = 2.0f / length(cross(mk3(e0.x, e0.y, e0.z), mk3(e1.x, e1.y, e1.z)));
+    r.pos = {wp.x, wp.y, wp.z};
+    return r;
+}
+
+LT_DEV float spot_falloff(float cos_from_light, float cos_width, float cos_falloff) {
+    if (cos_from_light < cos_width) return 0.0f;
+    if (cos_from_light >= cos_falloff) return 1.0f;
+    float d = (cos_from_light - cos_width) / (cos_falloff - cos_width);
+    return (d * d) * (d * d);
+}
+
+// commons.glsl:224-300 (+ LT_LIGHT_POINT).  rp = rand4.
+LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,
+                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec) {
+    rec.light_idx = (uint32_t)(rp.x * (float)sv.num_lights);
+    const lt_light& L = sv.lights[rec.light_idx];
+    const uint32_t flags = L.light_flags;
+    const uint32_t lt = flags & 7u;
+    f3 Le = mk3(0.0f);
+    rec.flags = flags;
+    pdf_pos_w = 0.0f;
+    wi = mk3(0.0f);
+    wi_len = 0.0f;
+    pdf_pos_a = 0.0f;
+    cos_from_light = 0.0f;
+    if (lt == LT_LIGHT_AREA) {
+        const lt_prim_mesh_info pi = sv.prims[L.prim_mesh_idx];
+        rec.triangle_idx = (uint32_t)(rp.y * (float)L.num_triangles);
+        TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, rec.triangle_idx, L.world_matrix, rec.bary);
+        Mat lm = load_material(sv, pi.material_index, rec.bary);
+        wi = r.pos - p;
+        const float d2 = dot(wi, wi);
+        wi_len = sqrtf(d2);
+        wi /= wi_len;
+        cos_from_light = fabsf(dot(r.n_s, -wi));
+        Le = lm.emissive;
+        pdf_pos_a = r.triangle_pdf;
+        pdf_pos_w = pdf_pos_a * d2 / cos_from_light;
+        rec.instance_idx = L.prim_mesh_idx;
+    } else if (lt == LT_LIGHT_SPOT) {
+        wi = ld3(L.pos) - p;
+        const float d2 = dot(wi, wi);
+        wi_len = sqrtf(d2);
+        wi /= wi_len;
+        const f3 ld = normalize(ld3(L.to) - ld3(L.pos));
+        cos_from_light = dot(-wi, ld);
+        const float fo = spot_falloff(cos_from_light, sv.cones[2 * rec.light_idx], sv.cones[2 * rec.light_idx + 1]);
+        pdf_pos_a = 1.0f;
+        pdf_pos_w = d2;
+        Le = ld3(L.L) * fo;
+    } else if (lt == LT_LIGHT_DIRECTIONAL) {
+        const f3 dir = normalize(ld3(L.pos) - ld3(L.to));
+        const f3 lp = p + dir * (2.0f * L.world_radius);
+        wi = lp - p;
+        wi_len = length(wi);
+        wi /= wi_len;
+        pdf_pos_a = 1.0f;
+        pdf_pos_w = 1.0f;
+        Le = ld3(L.L);
+        cos_from_light = 1.0f;
+    } else if (lt == LT_LIGHT_POINT) {
+        wi = ld3(L.pos) - p;
+        const float d2 = dot(wi, wi);
+        wi_len = sqrtf(d2);
+        wi /= wi_len;
+        cos_from_light = 1.0f;
+        pdf_pos_a = 1.0f;
+        pdf_pos_w = d2;
+        Le = ld3(L.L);
+    }
+    return Le;
+}
+
+}  // namespace ltd

@@ -1,0 +1,254 @@
+// integrator_abi.cpp — integrator objects of the C ABI (lt_integrator_*,
+// lt_render*, lt_readback_rgba32f, lt_image_sq_error) and the scene entry
+// points (lt_scene_*).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "context.h"
+#include "path.h"
+#include "scene.h"
+
+lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, size_t n, double* d_out);
+
+struct lt_integrator {
+    lt_context* ctx = nullptr;
+    lt_integrator_kind kind = LT_INTEGRATOR_PATH;
+    lt_path_config path{};
+    float4* accum = nullptr;  // output_tex, RGBA32F, row-major
+    int counting = 0;
+    lt_render_stats last{};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool rendered = false;
+};
+
+extern "C" {
+
+lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const void* cfg, size_t cfg_size,
+                               lt_integrator** out) {
+    if (!ctx || !cfg || !out) return LT_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (kind != LT_INTEGRATOR_PATH)
+        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "only the Path integrator is built in this version");
+    if (cfg_size != sizeof(lt_path_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
+    const lt_path_config* pc = (const lt_path_config*)cfg;
+    if (pc->width == 0 || pc->height == 0 || pc->width > 16384 || pc->height > 16384)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "image size out of range");
+    if (pc->strategy != LT_PATH_MEGAKERNEL)
+        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "path strategy not built");
+    lt_integrator* it = new (std::nothrow) lt_integrator;
+    if (!it) return LT_ERR_OUT_OF_MEMORY;
+    it->ctx = ctx;
+    it->kind = kind;
+    it->path = *pc;
+    (void)hipSetDevice(ctx->device);
+    size_t bytes = (size_t)pc->width * pc->height * sizeof(float4);
+    hipError_t e = hipMalloc(&it->accum, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(it->accum, 0, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&it->e0);
+    if (e == hipSuccess) e = hipEventCreate(&it->e1);
+    if (e != hipSuccess) {
+        lt_status st = lt_hip_check(ctx, e, "lt_integrator_create");
+        lt_integrator_destroy(it);
+        return st;
+    }
+    *out = it;
+    return LT_OK;
+}
+
+lt_status lt_integrator_destroy(lt_integrator* it) {
+    if (!it) return LT_OK;
+    if (it->ctx) (void)hipStreamSynchronize(it->ctx->stream);
+    if (it->accum) (void)hipFree(it->accum);
+    if (it->e0) (void)hipEventDestroy(it->e0);
+    if (it->e1) (void)hipEventDestroy(it->e1);
+    delete it;
+    return LT_OK;
+}
+
+lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
+    if (!it || !p) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = it->ctx;
+    if (!ctx->bvh.built) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_render before lt_accel_build");
+    if (ctx->scene.nlights == 0 || ctx->scene.total_light_triangle_cnt == 0)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene has no lights");
+    (void)hipSetDevice(ctx->device);
+    PathLaunch L{};
+    memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+    memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+    L.width = it->path.width;
+    L.height = it->path.height;
+    L.max_depth = it->path.max_depth;
+    memcpy(L.sky_col, it->path.sky_col, 12);
+    L.direct_lighting = it->path.direct_lighting;
+    L.frame_start = p->frame_num;
+    L.spp = p->spp ? p->spp : 1u;
+    L.seed = p->seed;
+    LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
+    lt_status st = launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+    if (st != LT_OK) return st;
+    LT_HIP(ctx, hipEventRecord(it->e1, ctx->stream));
+    it->last.frames = L.spp;
+    it->rendered = true;
+    return LT_OK;
+}
+
+lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* s) {
+    if (!it || !s) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = it->ctx;
+    if (!it->rendered) return lt_fail(ctx, LT_ERR_NOT_READY, "no render yet");
+    LT_HIP(ctx, hipEventSynchronize(it->e1));
+    float ms = 0.0f;
+    LT_HIP(ctx, hipEventElapsedTime(&ms, it->e0, it->e1));
+    it->last.kernel_ms = ms;
+    it->last.closest_rays = it->last.shadow_rays = 0;
+    if (it->counting) {
+        uint64_t h[2];
+        LT_HIP(ctx, hipMemcpyAsync(h, ctx->count_buf, 16, hipMemcpyDeviceToHost, ctx->stream));
+        LT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        it->last.closest_rays = h[0];
+        it->last.shadow_rays = h[1];
+    }
+    *s = it->last;
+    return LT_OK;
+}
+
+lt_status lt_render_set_counting(lt_integrator* it, int enable) {
+    if (!it) return LT_ERR_INVALID_ARG;
+    it->counting = enable ? 1 : 0;
+    return LT_OK;
+}
+
+lt_status lt_accum_reset(lt_integrator* it) {
+    if (!it) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = it->ctx;
+    LT_HIP(ctx, hipMemsetAsync(it->accum, 0, (size_t)it->path.width * it->path.height * sizeof(float4), ctx->stream));
+    return LT_OK;
+}
+
+lt_status lt_readback_rgba32f(lt_integrator* it, float* host, size_t n_floats) {
+    if (!it || !host) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = it->ctx;
+    size_t need = (size_t)it->path.width * it->path.height * 4;
+    if (n_floats < need) return lt_fail(ctx, LT_ERR_INVALID_ARG, "readback buffer too small");
+    LT_HIP(ctx, hipMemcpyAsync(host, it->accum, need * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LT_OK;
+}
+
+lt_status lt_image_sq_error(lt_context* ctx, const float* a, const float* b, size_t n_pixels, uint32_t flags,
+                            double* sum_sq) {
+    if (!ctx || !sum_sq || (n_pixels && (!a || !b))) return LT_ERR_INVALID_ARG;
+    *sum_sq = 0.0;
+    if (n_pixels == 0) return LT_OK;
+    (void)hipSetDevice(ctx->device);
+    const float4 *da = (const float4*)a, *db = (const float4*)b;
+    float4 *ta = nullptr, *tb = nullptr;
+    double* dout = nullptr;
+    lt_status st = LT_OK;
+    auto cleanup = [&]() {
+        if (ta) (void)hipFree(ta);
+        if (tb) (void)hipFree(tb);
+        if (dout) (void)hipFree(dout);
+    };
+    hipError_t e = hipMalloc(&dout, sizeof(double));
+    if (e == hipSuccess && !(flags & LT_PTR_DEVICE)) {
+        e = hipMalloc(&ta, n_pixels * sizeof(float4));
+        if (e == hipSuccess) e = hipMalloc(&tb, n_pixels * sizeof(float4));
+        if (e == hipSuccess) e = hipMemcpyAsync(ta, a, n_pixels * sizeof(float4), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tb, b, n_pixels * sizeof(float4), hipMemcpyHostToDevice, ctx->stream);
+        da = ta;
+        db = tb;
+    }
+    if (e != hipSuccess) {
+        st = lt_hip_check(ctx, e, "lt_image_sq_error");
+        cleanup();
+        return st;
+    }
+    st = launch_sq_error(ctx, da, db, n_pixels, dout);
+    if (st == LT_OK) {
+        e = hipMemcpyAsync(sum_sq, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = lt_hip_check(ctx, e, "lt_image_sq_error");
+    }
+    cleanup();
+    return st;
+}
+
+// ------------------------------------------------------------- scenes ----
+lt_status lt_scene_procedural(lt_procedural_kind kind, const lt_procedural_params* params, lt_host_scene** out) {
+    if (!out) return LT_ERR_INVALID_ARG;
+    try {
+        return lt::make_procedural(kind, params, out);
+    } catch (const std::bad_alloc&) {
+        return LT_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return LT_ERR_INVALID_ARG;
+    }
+}
+
+lt_status lt_scene_load_json(const char* path, lt_host_scene** out, char* err, size_t err_len) {
+    if (!path || !out) return LT_ERR_INVALID_ARG;
+    std::string e;
+    lt_status st;
+    try {
+        st = lt::load_json_scene(path, out, e);
+    } catch (const std::bad_alloc&) {
+        st = LT_ERR_OUT_OF_MEMORY;
+        e = "out of memory";
+    } catch (const std::exception& ex) {
+        st = LT_ERR_PARSE;
+        e = ex.what();
+    }
+    if (err && err_len) {
+        strncpy(err, e.c_str(), err_len - 1);
+        err[err_len - 1] = 0;
+    }
+    return st;
+}
+
+lt_status lt_scene_free(lt_host_scene* s) {
+    delete s;
+    return LT_OK;
+}
+
+lt_status lt_scene_view(const lt_host_scene* s, lt_scene_desc* d) {
+    if (!s || !d) return LT_ERR_INVALID_ARG;
+    memset(d, 0, sizeof(*d));
+    d->vertices = s->vertices.data();
+    d->num_vertices = (uint32_t)s->vertices.size();
+    d->indices = s->indices.data();
+    d->num_indices = (uint32_t)s->indices.size();
+    d->prim_infos = s->prims.data();
+    d->num_prim_infos = (uint32_t)s->prims.size();
+    d->materials = s->materials.data();
+    d->num_materials = (uint32_t)s->materials.size();
+    d->lights = s->lights.data();
+    d->num_lights = (uint32_t)s->lights.size();
+    d->textures = s->textures.data();
+    d->num_textures = (uint32_t)s->textures.size();
+    d->roughness_texture_ids = s->rough_tex.empty() ? nullptr : s->rough_tex.data();
+    d->spot_cones = s->spot_cones.empty() ? nullptr : s->spot_cones.data();
+    d->total_light_triangle_cnt = s->total_light_triangle_cnt;
+    d->total_light_area = s->total_light_area;
+    d->dir_light_idx = s->dir_light_idx;
+    memcpy(d->scene_min, s->smin, 12);
+    memcpy(d->scene_max, s->smax, 12);
+    return LT_OK;
+}
+
+lt_status lt_scene_camera(const lt_host_scene* s, uint32_t width, uint32_t height, lt_camera* out) {
+    if (!s || !out || !width || !height) return LT_ERR_INVALID_ARG;
+    lt::make_camera(s->cam_fov, s->cam_pos, s->cam_dir, (float)width / (float)height, out);
+    return LT_OK;
+}
+
+lt_status lt_scene_integrator(const lt_host_scene* s, lt_scene_integrator_cfg* out) {
+    if (!s || !out) return LT_ERR_INVALID_ARG;
+    *out = s->icfg;
+    return LT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// metrics.hip — squared-error reduction behind lt_image_sq_error.
+//
+// The reference reduces sum_px |gt.rgb - out.rgb|^2 with subgroup adds in
+// calc_rmse.comp:20-41 / reduce_rmse.comp and divides in output_rmse.comp:23.
+// Here one pass: each thread accumulates a grid-stride slice in double, a
+// wave64 shuffle tree reduces it, one double atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+
+namespace {
+
+__global__ void k_zero_d(double* p) { *p = 0.0; }
+
+__global__ __launch_bounds__(256) void k_sq_error(const float4* __restrict__ a, const float4* __restrict__ b,
+                                                  size_t n, double* __restrict__ out) {
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 x = a[i], y = b[i];
+        const float dr = x.x - y.x, dg = x.y - y.y, db = x.z - y.z;
+        acc += (double)(dr * dr) + (double)(dg * dg) + (double)(db * db);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+}  // namespace
+
+lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, size_t n, double* d_out) {
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, ctx->stream, d_out);
+    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 8);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_sq_error, dim3(blocks), dim3(256), 0, ctx->stream, a, b, n, d_out);
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}

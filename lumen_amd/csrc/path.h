@@ -1,0 +1,20 @@
+// path.h — launch parameters of the Path integrator kernels.
+#pragma once
+#include "context.h"
+
+// Push-constant equivalent (PCPath, src/shaders/integrators/path/path_commons.h:3-15)
+// plus the camera matrices of the SceneUBO.
+struct PathLaunch {
+    float inv_view[16];
+    float inv_projection[16];
+    uint32_t width, height;
+    int32_t max_depth;
+    float sky_col[3];
+    uint32_t direct_lighting;
+    uint32_t frame_start;  // frame_num of the first frame of this launch
+    uint32_t spp;          // frames accumulated by this launch
+    uint32_t seed;         // XOR'ed into the RNG frame word
+};
+
+lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,
+                                 uint64_t* d_counts);

@@ -1,0 +1,225 @@
+// path.hip — the Path integrator on gfx950 (megakernel form).
+//
+// One thread per pixel, persistent waves pulling 8x8 pixel tiles, all frames
+// of a launch accumulated in registers with the reference's per-frame running
+// mean (path.rgen:106-112) so the result equals frame-by-frame launches.
+// Per-sample logic: src/shaders/integrators/path/path.rgen:26-104 and
+// src/shaders/integrators/pt_commons.glsl:3-47.
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+#include "dev_scene.h"
+#include "dev_trace.h"
+#include "path.h"
+
+using namespace ltd;
+
+namespace {
+
+constexpr float PATH_TMIN = 0.001f;    // path.rgen:19
+constexpr float PATH_TMAX = 10000.0f;  // path.rgen:20
+constexpr int RR_MIN_DEPTH = 3;        // path.rgen:21
+
+struct Ctr {
+    uint32_t closest, shadow;
+};
+
+template <bool COUNT>
+LT_DEV void trace_closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl, Ctr& c) {
+    if (COUNT) c.closest++;
+    HitRec h;
+    uint32_t dn = 0, dt = 0;
+    if (trace_bvh<false, false>(sv, o, d, PATH_TMIN, PATH_TMAX, h, stk, dn, dt))
+        fill_payload(sv, h.inst, h.prim, h.u, h.v, pl);
+    else
+        pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
+}
+
+// pt_commons.glsl:3-42
+template <bool COUNT>
+LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f3 pos, bool side, f3 n_s, f3 wo,
+                               Payload& pl, const TraceStack& stk, Ctr& c) {
+    f3 res = mk3(0.0f);
+    f3 wi;
+    float wi_len, pdf_light_w, pdf_light_a, cos_from_light;
+    LightRecord rec;
+    rec.triangle_idx = 0;
+    rec.instance_idx = 0;
+    f4 r4;
+    r4.x = rnd(seed);
+    r4.y = rnd(seed);
+    r4.z = rnd(seed);
+    r4.w = rnd(seed);
+    const f3 Le = sample_light_Li(sv, r4, pos, pdf_light_w, wi, wi_len, pdf_light_a, cos_from_light, rec);
+    const f3 p = offset_ray2(pos, n_s);
+    float bsdf_pdf, bsdf_rev;
+    float cos_x = dot(n_s, wi);
+    f3 f = eval_bsdf(n_s, wo, mat, side, wi, bsdf_pdf, bsdf_rev, false);
+    if (COUNT) c.shadow++;
+    HitRec h;
+    uint32_t dn = 0, dt = 0;
+    const bool visible = !trace_bvh<true, false>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, dn, dt);
+    if (visible && pdf_light_w > 0.0f) {
+        const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bsdf_pdf / pdf_light_w);
+        res += mis * f * fabsf(cos_x) * Le / pdf_light_w;
+    }
+    if ((rec.flags & 7u) == LT_LIGHT_AREA) {
+        const float r0 = rnd(seed), r1 = rnd(seed), r2 = rnd(seed);
+        f = sample_bsdf(n_s, wo, mat, 1, side, wi, bsdf_pdf, cos_x, r0, r1, r2);
+        if (bsdf_pdf != 0.0f) {
+            trace_closest<COUNT>(sv, p, wi, stk, pl, c);
+            if (pl.triangle_idx == rec.triangle_idx && pl.instance_idx == rec.instance_idx) {
+                const float wl = length(pl.pos - pos);
+                const float g = fabsf(dot(pl.n_s, -wi)) / (wl * wl);
+                const float mis = 1.0f / (1.0f + pdf_light_a / (g * bsdf_pdf));
+                res += f * mis * fabsf(cos_x) * Le / bsdf_pdf;
+            }
+        }
+    }
+    return res;
+}
+
+// path.rgen:26-104; returns false when the NaN guard drops the sample
+template <bool COUNT>
+LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word,
+                        const TraceStack& stk, f3& col, Ctr& c) {
+    RNG seed = {x, y, frame_word, 0u};
+    const float r0 = rnd(seed);
+    const float r1 = rnd(seed);
+    const float jx = r0 - 0.5f, jy = r1 - 0.5f;
+    const float ux = ((float)x + 0.5f + jx) / (float)L.width, uy = ((float)y + 0.5f + jy) / (float)L.height;
+    const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    const f4 o4 = mul4(L.inv_view, f4{0.0f, 0.0f, 0.0f, 1.0f});
+    const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
+    const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
+    const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
+    f3 origin = {o4.x, o4.y, o4.z};
+    f3 direction = {d4.x, d4.y, d4.z};
+    col = mk3(0.0f);
+    bool last_specular = false;
+    f3 throughput = mk3(1.0f);
+    Payload pl;
+    pl.valid = false;
+    pl.triangle_idx = pl.instance_idx = 0xFFFFFFFFu;
+    pl.pos = pl.n_s = pl.n_g = mk3(0.0f);
+    for (int depth = 0;; depth++) {
+        trace_closest<COUNT>(sv, origin, direction, stk, pl, c);
+        if (!pl.valid) {
+            if (depth > 0 || L.direct_lighting == 1) col += throughput * ld3(L.sky_col);
+            break;
+        }
+        const Payload hp = pl;
+        const Mat hit_mat = load_material(sv, hp.material_idx, hp.uv);
+        if ((depth == 0 && L.direct_lighting == 1) || last_specular) col += throughput * hit_mat.emissive;
+        if (depth >= L.max_depth - 1) break;
+        const f3 wo = -direction;
+        f3 n_s = hp.n_s;
+        bool side = true;
+        f3 n_g = hp.n_g;
+        if (dot(hp.n_g, wo) < 0.0f) n_g = -n_g;
+        if (dot(n_g, hp.n_s) < 0.0f) {
+            n_s = -n_s;
+            side = false;
+        }
+        origin = offset_ray(hp.pos, n_g);
+        last_specular = is_specular(hit_mat);
+        if (!last_specular) {
+            const float light_pick_pdf = 1.0f / (float)sv.total_light_triangle_cnt;
+            if (depth > 0 || L.direct_lighting == 1)
+                col += throughput * uniform_sample_light<COUNT>(sv, seed, hit_mat, hp.pos, side, n_s, wo, pl, stk, c) /
+                       light_pick_pdf;
+        }
+        float pdf, cos_theta;
+        const float s0 = rnd(seed), s1 = rnd(seed), s2 = rnd(seed);
+        const f3 f = sample_bsdf(n_s, wo, hit_mat, 1, side, direction, pdf, cos_theta, s0, s1, s2);
+        if (pdf == 0.0f) break;
+        throughput *= f * fabsf(cos_theta) / pdf;
+        float rr_scale = 1.0f;
+        if (hit_mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) rr_scale *= side ? 1.0f / hit_mat.ior : hit_mat.ior;
+        if (depth > RR_MIN_DEPTH) {
+            const float rr_prob = fminf(0.95f, luminance(throughput) * rr_scale);
+            if (rr_prob == 0.0f || rr_prob < rnd(seed))
+                break;
+            else
+                throughput /= rr_prob;
+        }
+    }
+    return !isnan(luminance(col));
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLaunch L, float4* __restrict__ accum,
+                                                           uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+                                                           unsigned long long* __restrict__ counts) {
+    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tiles_x = (L.width + 7) / 8, tiles_y = (L.height + 7) / 8;
+    const uint32_t n_tiles = tiles_x * tiles_y;
+    Ctr c{0, 0};
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        t = __shfl(t, 0);
+        if (t >= n_tiles) break;
+        const uint32_t x = (t % tiles_x) * 8 + (lane & 7), y = (t / tiles_x) * 8 + (lane >> 3);
+        if (x >= L.width || y >= L.height) continue;
+        const size_t pix = (size_t)y * L.width + x;
+        float4 acc = accum[pix];
+        for (uint32_t f = 0; f < L.spp; f++) {
+            const uint32_t frame_num = L.frame_start + f;
+            f3 col;
+            if (!path_sample<COUNT>(sv, L, x, y, frame_num ^ L.seed, stk, col, c)) continue;  // path.rgen:102-104
+            if (frame_num > 0) {
+                const float w = 1.0f / (float)(frame_num + 1);
+                acc.x = acc.x * (1.0f - w) + col.x * w;
+                acc.y = acc.y * (1.0f - w) + col.y * w;
+                acc.z = acc.z * (1.0f - w) + col.z * w;
+            } else {
+                acc.x = col.x;
+                acc.y = col.y;
+                acc.z = col.z;
+            }
+            acc.w = 1.0f;
+        }
+        accum[pix] = acc;
+    }
+    if (COUNT) {
+        unsigned long long a = c.closest, b = c.shadow;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&counts[0], a);
+            atomicAdd(&counts[1], b);
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace
+
+lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,
+                                 uint64_t* d_counts) {
+    const size_t blocks = (size_t)ctx->num_cus * 4;
+    lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
+    if (st != LT_OK) return st;
+    SceneView sv = make_scene_view(ctx);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 4u);
+    if (count) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, 4u);
+    auto* cnt = (unsigned long long*)d_counts;
+    if (count)
+        hipLaunchKernelGGL((k_path_mega<true>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
+                           ctx->counters, ctx->spill, cnt);
+    else
+        hipLaunchKernelGGL((k_path_mega<false>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
+                           ctx->counters, ctx->spill, cnt);
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}

@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker / the timed CPU baseline; the product
+(lumen_amd/) never does.  Build with ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
+    return LIB_PATH
+
+
+class PathParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
+                ("sky_col", C.c_float * 3), ("direct_lighting", C.c_uint32), ("frame_start", C.c_uint32),
+                ("frames", C.c_uint32), ("seed", C.c_uint32), ("x0", C.c_uint32), ("y0", C.c_uint32),
+                ("x1", C.c_uint32), ("y1", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        F = C.POINTER(C.c_float)
+        L.ora_pcg4d.argtypes = [P, P, C.c_size_t]
+        L.ora_uint_to_float.argtypes = [C.c_uint32]
+        L.ora_uint_to_float.restype = C.c_float
+        L.ora_rand_seq.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_size_t, P]
+        L.ora_offset_ray.argtypes = [P, P, P, C.c_int]
+        L.ora_concentric_disk.argtypes = [P, P]
+        L.ora_onb.argtypes = [P, P, P]
+        L.ora_sample_bsdf.argtypes = [P, P, P, C.c_uint32, C.c_int, P, P, P, F, F]
+        L.ora_eval_bsdf.argtypes = [P, P, P, P, C.c_uint32, C.c_int, C.c_int, P, F, F]
+        L.ora_bsdf_pdf.argtypes = [P, P, P, P, C.c_int]
+        L.ora_bsdf_pdf.restype = C.c_float
+        L.ora_fresnel_dielectric.argtypes = [C.c_float, C.c_float, C.c_int]
+        L.ora_fresnel_dielectric.restype = C.c_float
+        L.ora_fresnel_conductor.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.ora_fresnel_conductor.restype = C.c_float
+        L.ora_D_ggx_iso.argtypes = [C.c_float, C.c_float]
+        L.ora_D_ggx_iso.restype = C.c_float
+        L.ora_sample_light_Li.argtypes = [P, P, P, P]
+        L.ora_accel_build.argtypes = [P]
+        L.ora_accel_build.restype = P
+        L.ora_accel_free.argtypes = [P]
+        L.ora_trace_closest.argtypes = [P, P, C.c_size_t, P, C.c_int, C.c_int]
+        L.ora_trace_any.argtypes = [P, P, C.c_size_t, P, C.c_int, C.c_int]
+        L.ora_payload.argtypes = [P, P, P, P]
+        L.ora_render_path.argtypes = [P, P, P, C.POINTER(PathParams), P, C.c_int, C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+        L.ora_render_path.restype = C.c_double
+        L.ora_hardware_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f3(v):
+    return np.ascontiguousarray(v, np.float32)
+
+
+def pcg4d(v: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(v, np.uint32).reshape(-1, 4)
+    out = np.zeros_like(v)
+    lib().ora_pcg4d(v.ctypes.data, out.ctypes.data, len(v))
+    return out
+
+
+def rand_seq(x, y, frame, n, w0=0) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().ora_rand_seq(x, y, frame, w0, n, out.ctypes.data)
+    return out
+
+
+def offset_ray(p, n, which=1) -> np.ndarray:
+    p, n = _f3(p), _f3(n)
+    out = np.zeros(3, np.float32)
+    lib().ora_offset_ray(p.ctypes.data, n.ctypes.data, out.ctypes.data, which)
+    return out
+
+
+def concentric_disk(r) -> np.ndarray:
+    r = _f3(r)
+    out = np.zeros(2, np.float32)
+    lib().ora_concentric_disk(r.ctypes.data, out.ctypes.data)
+    return out
+
+
+def onb(n):
+    n = _f3(n)
+    b1, b2 = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().ora_onb(n.ctypes.data, b1.ctypes.data, b2.ctypes.data)
+    return b1, b2
+
+
+def sample_bsdf(mat, n_s, wo, rands, mode=1, side=True):
+    n_s, wo, rands = _f3(n_s), _f3(wo), _f3(rands)
+    wi, f = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    pdf, cos = C.c_float(), C.c_float()
+    lib().ora_sample_bsdf(C.byref(mat), n_s.ctypes.data, wo.ctypes.data, mode, int(side), rands.ctypes.data,
+                          wi.ctypes.data, f.ctypes.data, C.byref(pdf), C.byref(cos))
+    return wi, f, pdf.value, cos.value
+
+
+def eval_bsdf(mat, n_s, wo, wi, mode=1, side=True, rev=True):
+    n_s, wo, wi = _f3(n_s), _f3(wo), _f3(wi)
+    f = np.zeros(3, np.float32)
+    pdf, pdf_rev = C.c_float(), C.c_float()
+    lib().ora_eval_bsdf(C.byref(mat), n_s.ctypes.data, wo.ctypes.data, wi.ctypes.data, mode, int(side), int(rev),
+                        f.ctypes.data, C.byref(pdf), C.byref(pdf_rev))
+    return f, pdf.value, pdf_rev.value
+
+
+def bsdf_pdf(mat, n_s, wo, wi, side=True) -> float:
+    n_s, wo, wi = _f3(n_s), _f3(wo), _f3(wi)
+    return lib().ora_bsdf_pdf(C.byref(mat), n_s.ctypes.data, wo.ctypes.data, wi.ctypes.data, int(side))
+
+
+def sample_light_Li(desc, rands4, p):
+    rands4, p = _f3(rands4), _f3(p)
+    out = np.zeros(12, np.float32)
+    lib().ora_sample_light_Li(C.byref(desc), rands4.ctypes.data, p.ctypes.data, out.ctypes.data)
+    return out
+
+
+class Accel:
+    """Host BVH2 over the scene's triangles (+ brute force)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self._h = lib().ora_accel_build(C.byref(desc))
+
+    def closest(self, rays: np.ndarray, brute=False, threads=0) -> np.ndarray:
+        from lumen_amd._abi import HIT_DTYPE, RAY_DTYPE  # record layouts only
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        lib().ora_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data, int(brute),
+                                threads or lib().ora_hardware_threads())
+        return hits
+
+    def any(self, rays: np.ndarray, brute=False, threads=0) -> np.ndarray:
+        from lumen_amd._abi import RAY_DTYPE
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint8)
+        lib().ora_trace_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, int(brute),
+                            threads or lib().ora_hardware_threads())
+        return occ
+
+    def payload(self, hit, ray) -> np.ndarray:
+        out = np.zeros(13, np.float32)
+        h = np.ascontiguousarray(hit)
+        r = np.ascontiguousarray(ray)
+        lib().ora_payload(C.byref(self.desc), h.ctypes.data, r.ctypes.data, out.ctypes.data)
+        return out
+
+    def render_path(self, camera, width, height, max_depth, sky_col=(0, 0, 0), direct_lighting=1, frame_start=0,
+                    frames=1, seed=0, window=None, accum=None, threads=0):
+        """Runs the reference Path integrator on the CPU; returns (image, seconds, closest, shadow)."""
+        p = PathParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.sky_col[:] = list(sky_col)
+        p.direct_lighting = direct_lighting
+        p.frame_start, p.frames, p.seed = frame_start, frames, seed
+        if window:
+            p.x0, p.y0, p.x1, p.y1 = window
+        if accum is None:
+            accum = np.zeros((height, width, 4), np.float32)
+        cl, sh = C.c_uint64(), C.c_uint64()
+        secs = lib().ora_render_path(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), accum.ctypes.data,
+                                     threads or lib().ora_hardware_threads(), C.byref(cl), C.byref(sh))
+        return accum, secs, cl.value, sh.value
+
+    def __del__(self):
+        try:
+            lib().ora_accel_free(self._h)
+        except Exception:
+            pass
+
+
+def bvh8_trace(nodes: np.ndarray, tris: np.ndarray, rays: np.ndarray, any_hit: bool = False):
+    """Host traversal of an exported device BVH8: (hits or occluded, node counts, tri counts)."""
+    from lumen_amd._abi import HIT_DTYPE, RAY_DTYPE  # record layouts only
+    L = lib()
+    P = C.c_void_p
+    L.ora_bvh8_trace.argtypes = [P, C.c_uint32, P, P, C.c_size_t, P, P, C.c_int, P, P]
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    hits = np.zeros(n, HIT_DTYPE)
+    occ = np.zeros(n, np.uint8)
+    nc = np.zeros(n, np.uint64)
+    tc = np.zeros(n, np.uint64)
+    nodes = np.ascontiguousarray(nodes)
+    tris = np.ascontiguousarray(tris)
+    L.ora_bvh8_trace(nodes.ctypes.data, len(nodes), tris.ctypes.data, rays.ctypes.data, n, hits.ctypes.data,
+                     occ.ctypes.data, int(any_hit), nc.ctypes.data, tc.ctypes.data)
+    return (occ if any_hit else hits), nc, tc
+
+
+def hardware_threads() -> int:
+    return lib().ora_hardware_threads()
