@@ -1,0 +1,217 @@
+"""Headline benchmark: Path integrator throughput (Mrays/s) at 1920x1080 on
+one MI355X, on the procedural terrain workload (1.0M-triangle fBm heightfield
++ 2,000 instanced boxes, Disney materials, 64 point lights + 1 area light,
+Russian roulette after depth 3, max depth 12).
+
+A step is one 1-spp frame of the whole image (one lt_render launch).  Prints
+one JSON line with the metric, the roofline of the path kernel (algorithmic
+BVH bytes per launch = nodes_visited * 128 B + tris_tested * 48 B, counted by
+the kernels themselves in an untimed counting pass over the same frames,
+divided by the kernel's HIP-event duration) and the CPU baseline (the oracle
+restatement of the reference Path integrator on the host cores, on a band of
+rows of the same frame).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(workload: str):
+    """HBM bytes per launch from the committed rocprof counter summary, if any."""
+    p = ROOT / "profiles" / "path_megakernel_terrain1080p_pmc.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        if d.get("workload") == workload:
+            return d.get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+    return None
+
+
+def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
+    import numpy as np
+
+    import oracle
+
+    log("cpu baseline: building host BVH")
+    acc = oracle.Accel(scene.desc)
+    threads = oracle.hardware_threads()
+    # calibrate on a band of 32 rows, then size the sample (rows x frames) for
+    # about target_s seconds of CPU work
+    y0 = H // 2 - 16
+    _, secs, _, _ = acc.render_path(camera, W, H, depth, sky_col=sky, frame_start=frame, frames=1,
+                                    window=(0, y0, W, y0 + 32), threads=threads)
+    per_row = max(secs / 32, 1e-5)
+    if per_row * H >= target_s:
+        rows, frames = int(max(8, target_s / per_row)), 1
+    else:
+        rows, frames = H, int(max(1, min(64, target_s / (per_row * H))))
+    y0 = max(0, H // 2 - rows // 2)
+    log(f"cpu baseline: {rows} rows x {W} px x {frames} frame(s), {threads} threads")
+    _, secs, cl, sh = acc.render_path(camera, W, H, depth, sky_col=sky, frame_start=frame, frames=frames,
+                                      window=(0, y0, W, y0 + rows), threads=threads)
+    return {
+        "value": round((cl + sh) / secs / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"rows {y0}..{y0 + rows - 1} of the 1920x1080 frames {frame}..{frame + frames - 1}, 1 spp each "
+                   f"({cl + sh} rays, {secs:.2f} s)"),
+    }
+
+
+def stream_copy(mib: int) -> int:
+    """Copies mib MiB device-to-device 5 times with the plain copy kernel;
+    reports read+write GB/s (the attainable HBM bandwidth)."""
+    import lumen_amd as la
+
+    ctx = la.Context(0)
+    n = mib << 20
+    src, dst = ctx.alloc(n), ctx.alloc(n)
+    times = [ctx.stream_copy(dst, src, n) for _ in range(5)]
+    best = min(times)
+    print(json.dumps({"stream_copy_bytes": n, "ms": [round(t, 4) for t in times],
+                      "read_plus_write_GBps": round(2 * n / (best * 1e-3) / 1e9, 1)}), flush=True)
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+    return 0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stream-copy", type=int, default=0, metavar="MIB",
+                    help="only run the HBM copy kernel on MIB MiB (bandwidth / counter calibration)")
+    args = ap.parse_args()
+    if args.stream_copy:
+        return stream_copy(args.stream_copy)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box here has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import lumen_amd as la
+
+    W, H = args.width, args.height
+    log("generating terrain scene")
+    t0 = time.time()
+    scene = la.Scene.procedural("terrain", seed=1)
+    log(f"scene: {scene.num_triangles} triangles in {time.time() - t0:.1f}s")
+    ctx = la.Context(0)
+    n_cu, arch = ctx.device_info()
+    it = la.Path(scene, ctx, W, H)
+    it.init()
+    bst = it.create_accel()
+    log(f"bvh8: {bst.num_nodes} nodes, depth {bst.max_depth}, build {bst.build_ms:.1f} ms on {arch} ({n_cu} CUs)")
+    depth = it.path_length
+    sky = tuple(scene.integrator_cfg().sky_col)
+
+    for _ in range(args.warmup):
+        it.render(spp=1)
+        it.update()
+    ctx.synchronize()
+    first = it.frame_num
+
+    # timed region: exactly K steps, synchronised on both sides
+    kernel_ms = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it.render(spp=1)
+        kernel_ms.append(it.stats().kernel_ms)  # HIP events on the context stream
+        it.update()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} frames in {wall:.3f}s")
+
+    # untimed counting pass over the same frame indices (ray and BVH work counts
+    # are a function of the frame index only)
+    it.set_counting(True)
+    rays = nodes = tris = 0
+    for k in range(args.steps):
+        it.frame_num = first + k
+        it.render(spp=1)
+        s = it.stats()
+        rays += s.closest_rays + s.shadow_rays
+        nodes += s.nodes_visited
+        tris += s.tris_tested
+    it.set_counting(False)
+
+    mrays = rays / wall / 1e6
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    bytes_per_launch = (nodes * 128 + tris * 48) / args.steps
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    workload = "terrain-1M+2000-boxes-1920x1080-path-d12"
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
+
+    out = {
+        "metric": "Mrays/s (Path, 1080p, 1 GPU)",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {
+            "workload": workload,
+            "integrator": "path-megakernel",
+            "width": W,
+            "height": H,
+            "spp_per_step": 1,
+            "max_depth": depth,
+            "triangles": scene.num_triangles,
+            "rays_per_step": rays // args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(workload),
+            "kernel": "k_path_mega",
+            "kernel_ms_avg": round(avg_ms, 3),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+        },
+        "cpu_baseline": cpu,
+        "bvh_build_ms": round(bst.build_ms, 2),
+    }
+    print(json.dumps(out), flush=True)
+    it.destroy()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

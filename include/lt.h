@@ -259,6 +259,10 @@ lt_status lt_device_free(lt_context* ctx, void* dptr);
 lt_status lt_memcpy_h2d(lt_context* ctx, void* dst, const void* src, size_t bytes);
 lt_status lt_memcpy_d2h(lt_context* ctx, void* dst, const void* src, size_t bytes);
 lt_status lt_synchronize(lt_context* ctx);
+/* Plain device-to-device copy kernel (16-B vector loads and stores, grid
+ * stride).  No reference counterpart: it measures the attainable HBM bandwidth
+ * and calibrates the rocprof byte counters used for the roofline. */
+lt_status lt_stream_copy(lt_context* ctx, void* dst, const void* src, size_t bytes, float* ms);
 
 /* ---- integrators ------------------------------------------------------------ */
 
@@ -296,9 +300,11 @@ typedef struct lt_frame_params {
 
 /* Render statistics of the last lt_render call. */
 typedef struct lt_render_stats {
-    uint64_t closest_rays; /* closest-hit queries traced */
-    uint64_t shadow_rays;  /* occlusion queries traced   */
-    float kernel_ms;       /* device time of the render  */
+    uint64_t closest_rays;   /* closest-hit queries traced            */
+    uint64_t shadow_rays;    /* occlusion queries traced              */
+    uint64_t nodes_visited;  /* BVH8 nodes popped (128 B each)        */
+    uint64_t tris_tested;    /* triangle records tested (48 B each)   */
+    float kernel_ms;         /* device time of the render             */
     uint32_t frames;
 } lt_render_stats;
 
@@ -367,6 +373,9 @@ lt_status lt_scene_view(const lt_host_scene* s, lt_scene_desc* out);
  * W/H, dir, pos) then update_view_matrix (src/Framework/Camera.h:28-38,67-85,
  * 109-121; src/RayTracer/Integrator.cpp:60-72). */
 lt_status lt_scene_camera(const lt_host_scene* s, uint32_t width, uint32_t height, lt_camera* out);
+/* Replaces the JSON "camera" block (fov in degrees, position, dir;
+ * LumenScene.cpp:487-491). */
+lt_status lt_scene_set_camera(lt_host_scene* s, float fov, const float pos[3], const float dir[3]);
 /* Integrator block of the scene JSON (LumenScene.cpp:237-269): type
  * (lt_integrator_kind), path_length, sky_col, VCM enable_vm / radius_factor. */
 typedef struct lt_scene_integrator_cfg {

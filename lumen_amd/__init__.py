@@ -56,6 +56,11 @@ class Scene:
         check(lib().lt_scene_camera(self._h, width, height, C.byref(cam)), what="lt_scene_camera")
         return cam
 
+    def set_camera(self, fov: float, pos, direction) -> None:
+        p = (C.c_float * 3)(*pos)
+        d = (C.c_float * 3)(*direction)
+        check(lib().lt_scene_set_camera(self._h, fov, p, d), what="lt_scene_set_camera")
+
     def integrator_cfg(self) -> _abi.lt_scene_integrator_cfg:
         cfg = _abi.lt_scene_integrator_cfg()
         check(lib().lt_scene_integrator(self._h, C.byref(cfg)), what="lt_scene_integrator")
@@ -177,6 +182,12 @@ class Context:
     def last_kernel_ms(self) -> float:
         ms = C.c_float()
         check(lib().lt_last_kernel_ms(self._h, C.byref(ms)), self._h, "lt_last_kernel_ms")
+        return ms.value
+
+    def stream_copy(self, d_dst: int, d_src: int, nbytes: int) -> float:
+        """Device-to-device copy kernel; returns its duration in ms."""
+        ms = C.c_float()
+        check(lib().lt_stream_copy(self._h, d_dst, d_src, nbytes, C.byref(ms)), self._h, "lt_stream_copy")
         return ms.value
 
     def synchronize(self) -> None:

@@ -98,8 +98,8 @@ class lt_frame_params(C.Structure):
 
 
 class lt_render_stats(C.Structure):
-    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("kernel_ms", C.c_float),
-                ("frames", C.c_uint32)]
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
+                ("tris_tested", C.c_uint64), ("kernel_ms", C.c_float), ("frames", C.c_uint32)]
 
 
 class lt_procedural_params(C.Structure):
@@ -137,6 +137,7 @@ _PROTOS = {
     "lt_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),
     "lt_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
     "lt_synchronize": (C.c_int, [P]),
+    "lt_stream_copy": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_float)]),
     "lt_integrator_create": (C.c_int, [P, C.c_int, P, C.c_size_t, C.POINTER(P)]),
     "lt_integrator_destroy": (C.c_int, [P]),
     "lt_render": (C.c_int, [P, C.POINTER(lt_frame_params)]),
@@ -151,6 +152,7 @@ _PROTOS = {
     "lt_scene_view": (C.c_int, [P, C.POINTER(lt_scene_desc)]),
     "lt_scene_camera": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(lt_camera)]),
     "lt_scene_integrator": (C.c_int, [P, C.POINTER(lt_scene_integrator_cfg)]),
+    "lt_scene_set_camera": (C.c_int, [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 }
 EXPORTED = tuple(_PROTOS)
 

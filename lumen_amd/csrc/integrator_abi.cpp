@@ -103,13 +103,15 @@ lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* s) {
     float ms = 0.0f;
     LT_HIP(ctx, hipEventElapsedTime(&ms, it->e0, it->e1));
     it->last.kernel_ms = ms;
-    it->last.closest_rays = it->last.shadow_rays = 0;
+    it->last.closest_rays = it->last.shadow_rays = it->last.nodes_visited = it->last.tris_tested = 0;
     if (it->counting) {
-        uint64_t h[2];
-        LT_HIP(ctx, hipMemcpyAsync(h, ctx->count_buf, 16, hipMemcpyDeviceToHost, ctx->stream));
+        uint64_t h[4];
+        LT_HIP(ctx, hipMemcpyAsync(h, ctx->count_buf, 32, hipMemcpyDeviceToHost, ctx->stream));
         LT_HIP(ctx, hipStreamSynchronize(ctx->stream));
         it->last.closest_rays = h[0];
         it->last.shadow_rays = h[1];
+        it->last.nodes_visited = h[2];
+        it->last.tris_tested = h[3];
     }
     *s = it->last;
     return LT_OK;
@@ -242,6 +244,15 @@ lt_status lt_scene_view(const lt_host_scene* s, lt_scene_desc* d) {
 lt_status lt_scene_camera(const lt_host_scene* s, uint32_t width, uint32_t height, lt_camera* out) {
     if (!s || !out || !width || !height) return LT_ERR_INVALID_ARG;
     lt::make_camera(s->cam_fov, s->cam_pos, s->cam_dir, (float)width / (float)height, out);
+    return LT_OK;
+}
+
+lt_status lt_scene_set_camera(lt_host_scene* s, float fov, const float pos[3], const float dir[3]) {
+    if (!s || !pos || !dir || !(fov > 0.0f && fov < 180.0f)) return LT_ERR_INVALID_ARG;
+    if (dir[0] == 0.0f && dir[1] == 0.0f && dir[2] == 0.0f) return LT_ERR_INVALID_ARG;
+    s->cam_fov = fov;
+    memcpy(s->cam_pos, pos, 12);
+    memcpy(s->cam_dir, dir, 12);
     return LT_OK;
 }
 

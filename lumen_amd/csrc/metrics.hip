@@ -34,3 +34,29 @@ lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, siz
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;
 }
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" lt_status lt_stream_copy(lt_context* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+    if (!ctx || !dst || !src || (bytes & 15)) return LT_ERR_INVALID_ARG;
+    const size_t n = bytes / 16;
+    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 16);
+    if (blocks == 0) blocks = 1;
+    LT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)src, (uint4*)dst, n);
+    LT_HIP(ctx, hipGetLastError());
+    LT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    LT_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    LT_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
+    return LT_OK;
+}

@@ -21,15 +21,14 @@ constexpr float PATH_TMAX = 10000.0f;  // path.rgen:20
 constexpr int RR_MIN_DEPTH = 3;        // path.rgen:21
 
 struct Ctr {
-    uint32_t closest, shadow;
+    uint32_t closest, shadow, nodes, tris;
 };
 
 template <bool COUNT>
 LT_DEV void trace_closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl, Ctr& c) {
     if (COUNT) c.closest++;
     HitRec h;
-    uint32_t dn = 0, dt = 0;
-    if (trace_bvh<false, false>(sv, o, d, PATH_TMIN, PATH_TMAX, h, stk, dn, dt))
+    if (trace_bvh<false, COUNT>(sv, o, d, PATH_TMIN, PATH_TMAX, h, stk, c.nodes, c.tris))
         fill_payload(sv, h.inst, h.prim, h.u, h.v, pl);
     else
         pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
@@ -57,8 +56,7 @@ LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f
     f3 f = eval_bsdf(n_s, wo, mat, side, wi, bsdf_pdf, bsdf_rev, false);
     if (COUNT) c.shadow++;
     HitRec h;
-    uint32_t dn = 0, dt = 0;
-    const bool visible = !trace_bvh<true, false>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, dn, dt);
+    const bool visible = !trace_bvh<true, COUNT>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, c.nodes, c.tris);
     if (visible && pdf_light_w > 0.0f) {
         const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bsdf_pdf / pdf_light_w);
         res += mis * f * fabsf(cos_x) * Le / pdf_light_w;
@@ -157,7 +155,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLau
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t tiles_x = (L.width + 7) / 8, tiles_y = (L.height + 7) / 8;
     const uint32_t n_tiles = tiles_x * tiles_y;
-    Ctr c{0, 0};
+    Ctr c{0, 0, 0, 0};
     for (;;) {
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(work, 1u);
@@ -186,14 +184,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLau
         accum[pix] = acc;
     }
     if (COUNT) {
-        unsigned long long a = c.closest, b = c.shadow;
+        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, t = c.tris;
         for (int o = 32; o > 0; o >>= 1) {
             a += __shfl_xor(a, o);
             b += __shfl_xor(b, o);
+            n += __shfl_xor(n, o);
+            t += __shfl_xor(t, o);
         }
         if (lane == 0) {
             atomicAdd(&counts[0], a);
             atomicAdd(&counts[1], b);
+            atomicAdd(&counts[2], n);
+            atomicAdd(&counts[3], t);
         }
     }
 }
@@ -212,7 +214,7 @@ lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* a
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 4u);
-    if (count) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, 4u);
+    if (count) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, 8u);
     auto* cnt = (unsigned long long*)d_counts;
     if (count)
         hipLaunchKernelGGL((k_path_mega<true>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,

@@ -740,12 +740,14 @@ static lt_host_scene* make_terrain(const lt_procedural_params& p) {
         al.push_back(L);
     }
     finalize_scene(*s, al);
+    // Lumen's camera round trip (lookAtLH -> Euler XYZ -> T*Ry*Rx*Rz) looks
+    // along (-dir.x, dir.y, -dir.z): this views the terrain from above its +z edge
     s->cam_fov = 55.0f;
     s->cam_pos[0] = 0.0f;
-    s->cam_pos[1] = 28.0f;
-    s->cam_pos[2] = 62.0f;
+    s->cam_pos[1] = 40.0f;
+    s->cam_pos[2] = 58.0f;
     s->cam_dir[0] = 0.0f;
-    s->cam_dir[1] = 0.0f;
+    s->cam_dir[1] = -0.75f;
     s->cam_dir[2] = 1.0f;
     s->icfg.kind = LT_INTEGRATOR_PATH;
     s->icfg.path_length = 12;
