@@ -72,8 +72,16 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # headless C++ driver over the C ABI (host C++ only, links liblt.so)
+    cli = PKG / "lt_render_cli"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "host" / "lt_render_cli.cpp"),
+           f"-L{PKG}", "-llt", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(cli) + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(str(cli) + ".tmp", cli)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {cli}")
     return LIB
 
 

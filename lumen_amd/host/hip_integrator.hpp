@@ -1,0 +1,158 @@
+// hip_integrator.hpp — C++ host adapter: Lumen's Integrator contract over the
+// C ABI of include/lt.h (header-only, no HIP headers needed by the caller).
+//
+// Mirrors src/RayTracer/Integrator.h:14-33 (init / render / gui / update /
+// destroy / create_accel, public output_tex / updated / frame_num) and the
+// factory RayTracer::create_integrator (src/RayTracer/RayTracer.cpp:244-282).
+// Failures are turned into std::runtime_error, as LUMEN_ERROR does
+// (src/Framework/Logger.h:22-26); nothing throws across the C ABI itself.
+#pragma once
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lt.h"
+
+namespace lumen_amd {
+
+inline void check(lt_status st, lt_context* ctx, const char* what) {
+    if (st != LT_OK)
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string((int)st) +
+                                 "): " + (ctx ? lt_last_error(ctx) : ""));
+}
+
+// The host scene (LumenScene's flattened vectors) owned by the library.
+class LumenScene {
+   public:
+    explicit LumenScene(lt_host_scene* s) : s_(s) { check(lt_scene_view(s_, &desc_), nullptr, "lt_scene_view"); }
+    static std::unique_ptr<LumenScene> load(const std::string& json_path) {
+        lt_host_scene* s = nullptr;
+        char err[512] = {0};
+        lt_status st = lt_scene_load_json(json_path.c_str(), &s, err, sizeof(err));
+        if (st != LT_OK) throw std::runtime_error("lt_scene_load_json(" + json_path + "): " + err);
+        return std::unique_ptr<LumenScene>(new LumenScene(s));
+    }
+    static std::unique_ptr<LumenScene> procedural(lt_procedural_kind kind, uint32_t seed = 0) {
+        lt_procedural_params p{};
+        p.seed = seed;
+        lt_host_scene* s = nullptr;
+        check(lt_scene_procedural(kind, &p, &s), nullptr, "lt_scene_procedural");
+        return std::unique_ptr<LumenScene>(new LumenScene(s));
+    }
+    ~LumenScene() { lt_scene_free(s_); }
+    LumenScene(const LumenScene&) = delete;
+    LumenScene& operator=(const LumenScene&) = delete;
+    const lt_scene_desc& desc() const { return desc_; }
+    lt_camera camera(uint32_t w, uint32_t h) const {
+        lt_camera c;
+        check(lt_scene_camera(s_, w, h, &c), nullptr, "lt_scene_camera");
+        return c;
+    }
+    lt_scene_integrator_cfg config() const {
+        lt_scene_integrator_cfg c;
+        check(lt_scene_integrator(s_, &c), nullptr, "lt_scene_integrator");
+        return c;
+    }
+
+   private:
+    lt_host_scene* s_;
+    lt_scene_desc desc_{};
+};
+
+// Integrator::output_tex: an RGBA32F image readable on the host.
+struct OutputTex {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;  // row-major, 4 floats per texel
+};
+
+// The abstract plugin interface (src/RayTracer/Integrator.h:14-33).
+class Integrator {
+   public:
+    Integrator(LumenScene* scene, lt_context* ctx, uint32_t w, uint32_t h) : lumen_scene(scene), ctx_(ctx), w_(w), h_(h) {}
+    virtual ~Integrator() { destroy(); }
+    virtual void init() = 0;
+    virtual void render() = 0;
+    virtual bool gui() { return false; }
+    // Integrator::update + Path::update (Path.cpp:61-68): frame_num advances,
+    // and restarts from 0 when the camera / settings changed (`updated`).
+    virtual bool update() {
+        frame_num++;
+        bool was = updated;
+        if (updated) {
+            frame_num = 0;
+            updated = false;
+        }
+        return was;
+    }
+    virtual void destroy() {
+        if (it_) lt_integrator_destroy(it_);
+        it_ = nullptr;
+    }
+    virtual lt_build_stats create_accel() {
+        lt_build_stats s{};
+        check(lt_accel_build(ctx_, nullptr, &s), ctx_, "lt_accel_build");
+        return s;
+    }
+    // Reads output_tex back (synchronises).
+    const OutputTex& output_tex() {
+        tex_.width = w_;
+        tex_.height = h_;
+        tex_.rgba.resize((size_t)w_ * h_ * 4);
+        check(lt_readback_rgba32f(it_, tex_.rgba.data(), tex_.rgba.size()), ctx_, "lt_readback_rgba32f");
+        return tex_;
+    }
+    bool updated = false;
+    uint32_t frame_num = 0;
+    uint32_t seed = 0;
+
+   protected:
+    LumenScene* lumen_scene;
+    lt_context* ctx_;
+    uint32_t w_, h_;
+    lt_integrator* it_ = nullptr;
+    OutputTex tex_;
+};
+
+// The unidirectional path tracer (src/RayTracer/Path.{h,cpp}).
+class Path final : public Integrator {
+   public:
+    Path(LumenScene* scene, lt_context* ctx, uint32_t w, uint32_t h, uint32_t strategy = LT_PATH_MEGAKERNEL)
+        : Integrator(scene, ctx, w, h), strategy_(strategy) {}
+    void init() override {
+        check(lt_scene_upload(ctx_, &lumen_scene->desc()), ctx_, "lt_scene_upload");
+        lt_scene_integrator_cfg sc = lumen_scene->config();
+        lt_path_config cfg{};
+        cfg.width = w_;
+        cfg.height = h_;
+        cfg.max_depth = sc.path_length > 0 ? sc.path_length : 6;  // SceneConfig.h:16
+        for (int k = 0; k < 3; k++) cfg.sky_col[k] = sc.sky_col[k];
+        cfg.direct_lighting = 1;  // Path.h: direct_lighting = true
+        cfg.strategy = strategy_;
+        check(lt_integrator_create(ctx_, LT_INTEGRATOR_PATH, &cfg, sizeof(cfg), &it_), ctx_, "lt_integrator_create");
+        camera_ = lumen_scene->camera(w_, h_);
+        frame_num = 0;
+    }
+    void render() override {
+        lt_frame_params p{};
+        p.camera = camera_;
+        p.frame_num = frame_num;
+        p.seed = seed;
+        p.spp = 1;
+        check(lt_render(it_, &p), ctx_, "lt_render");
+    }
+
+   private:
+    uint32_t strategy_;
+    lt_camera camera_{};
+};
+
+// RayTracer::create_integrator, keyed by the IntegratorType names
+// (src/RayTracer/SceneConfig.h:13) that this core builds.
+inline std::unique_ptr<Integrator> create_integrator(const std::string& name, LumenScene* scene, lt_context* ctx,
+                                                     uint32_t w, uint32_t h) {
+    if (name == "path" || name == "Path") return std::unique_ptr<Integrator>(new Path(scene, ctx, w, h));
+    throw std::runtime_error("integrator '" + name + "' is not built");
+}
+
+}  // namespace lumen_amd
