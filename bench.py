@@ -103,6 +103,8 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--strategy", choices=["wavefront", "megakernel"], default="wavefront",
+                    help="Path execution form (persistent wavefront stages or one megakernel)")
     ap.add_argument("--stream-copy", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy kernel on MIB MiB (bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -122,7 +124,7 @@ def main() -> int:
     log(f"scene: {scene.num_triangles} triangles in {time.time() - t0:.1f}s")
     ctx = la.Context(0)
     n_cu, arch = ctx.device_info()
-    it = la.Path(scene, ctx, W, H)
+    it = la.Path(scene, ctx, W, H, strategy=args.strategy)
     it.init()
     bst = it.create_accel()
     log(f"bvh8: {bst.num_nodes} nodes, depth {bst.max_depth}, build {bst.build_ms:.1f} ms on {arch} ({n_cu} CUs)")
@@ -185,7 +187,7 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {
             "workload": workload,
-            "integrator": "path-megakernel",
+            "integrator": f"path-{args.strategy}",
             "width": W,
             "height": H,
             "spp_per_step": 1,
@@ -200,7 +202,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(workload),
-            "kernel": "k_path_mega",
+            "kernel": "k_path_mega" if args.strategy == "megakernel" else "one frame's launch sequence (extend/shade/shadow/mis per depth)",
             "kernel_ms_avg": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },

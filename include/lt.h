@@ -249,6 +249,17 @@ lt_status lt_trace_any(lt_context* ctx, const lt_ray* rays, size_t n, uint8_t* o
 lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_hit, uint32_t flags,
                          uint64_t* nodes, uint64_t* tris);
 
+/* Launch shapes of the persistent kernels (workgroups per compute unit; 0
+ * keeps the current value).  Set by the autotuner (lumen_amd/autotune.py).
+ * No reference counterpart: the reference's launch shapes are the driver's. */
+typedef struct lt_tuning {
+    uint32_t path_blocks_per_cu;   /* Path megakernel, 128-thread workgroups (default 8) */
+    uint32_t trace_blocks_per_cu;  /* raw queries, 128-thread workgroups (default 8)     */
+    uint32_t reserved[14];
+} lt_tuning;
+lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
+lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);
+
 /* Device-time of the last kernel launch issued by lt_trace_* / lt_render, in ms
  * (HIP events on the context stream). */
 lt_status lt_last_kernel_ms(lt_context* ctx, float* ms);

@@ -112,6 +112,17 @@ class Context:
         check(lib().lt_device_info(self._h, C.byref(n), arch, 64), self._h, "lt_device_info")
         return n.value, arch.value.decode()
 
+    def set_tuning(self, **kw) -> None:
+        t = _abi.lt_tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().lt_set_tuning(self._h, C.byref(t)), self._h, "lt_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = _abi.lt_tuning()
+        check(lib().lt_get_tuning(self._h, C.byref(t)), self._h, "lt_get_tuning")
+        return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+
     def upload(self, scene: Scene) -> None:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
         self.scene = scene

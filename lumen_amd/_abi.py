@@ -108,6 +108,11 @@ class lt_procedural_params(C.Structure):
                 ("tex_res", C.c_uint32), ("reserved", C.c_uint32 * 9)]
 
 
+class lt_tuning(C.Structure):
+    _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("reserved", C.c_uint32 * 14)]
+
+
 class lt_scene_integrator_cfg(C.Structure):
     _fields_ = [("kind", C.c_int32), ("path_length", C.c_int32), ("sky_col", C.c_float * 3),
                 ("enable_vm", C.c_int32), ("radius_factor", C.c_float)]
@@ -132,6 +137,8 @@ _PROTOS = {
     "lt_trace_count": (C.c_int, [P, P, C.c_size_t, C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint64)]),
     "lt_last_kernel_ms": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "lt_set_tuning": (C.c_int, [P, C.POINTER(lt_tuning)]),
+    "lt_get_tuning": (C.c_int, [P, C.POINTER(lt_tuning)]),
     "lt_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
     "lt_device_free": (C.c_int, [P, P]),
     "lt_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),

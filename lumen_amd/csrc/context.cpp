@@ -288,6 +288,21 @@ lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_
     return LT_OK;
 }
 
+lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
+    if (!ctx || !t) return LT_ERR_INVALID_ARG;
+    if (t->path_blocks_per_cu > 64 || t->trace_blocks_per_cu > 64)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "blocks per CU must be <= 64");
+    if (t->path_blocks_per_cu) ctx->tuning.path_blocks_per_cu = t->path_blocks_per_cu;
+    if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
+    return LT_OK;
+}
+
+lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t) {
+    if (!ctx || !t) return LT_ERR_INVALID_ARG;
+    *t = ctx->tuning;
+    return LT_OK;
+}
+
 lt_status lt_last_kernel_ms(lt_context* ctx, float* ms) {
     if (!ctx || !ms) return LT_ERR_INVALID_ARG;
     LT_HIP(ctx, hipEventSynchronize(ctx->ev1));

@@ -78,8 +78,9 @@ struct lt_context {
     size_t ray_buf_bytes = 0;
     void* hit_buf = nullptr;
     size_t hit_buf_bytes = 0;
-    uint64_t* count_buf = nullptr;  // [nodes, tris]
+    uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
+    lt_tuning tuning{8u, 8u, {}};
 };
 
 // helpers shared by the .cpp / .hip translation units

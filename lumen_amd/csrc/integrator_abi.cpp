@@ -9,6 +9,7 @@
 
 #include "context.h"
 #include "path.h"
+#include "path_wave.h"
 #include "scene.h"
 
 lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, size_t n, double* d_out);
@@ -18,6 +19,7 @@ struct lt_integrator {
     lt_integrator_kind kind = LT_INTEGRATOR_PATH;
     lt_path_config path{};
     float4* accum = nullptr;  // output_tex, RGBA32F, row-major
+    WaveBuffers wave;         // wavefront state (LT_PATH_WAVEFRONT)
     int counting = 0;
     lt_render_stats last{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -36,8 +38,10 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
     const lt_path_config* pc = (const lt_path_config*)cfg;
     if (pc->width == 0 || pc->height == 0 || pc->width > 16384 || pc->height > 16384)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "image size out of range");
-    if (pc->strategy != LT_PATH_MEGAKERNEL)
-        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "path strategy not built");
+    if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown path strategy");
+    if (pc->strategy == LT_PATH_WAVEFRONT && pc->max_depth > kWaveMaxDepth)
+        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "wavefront path length is limited to 32");
     lt_integrator* it = new (std::nothrow) lt_integrator;
     if (!it) return LT_ERR_OUT_OF_MEMORY;
     it->ctx = ctx;
@@ -62,6 +66,7 @@ lt_status lt_integrator_destroy(lt_integrator* it) {
     if (!it) return LT_OK;
     if (it->ctx) (void)hipStreamSynchronize(it->ctx->stream);
     if (it->accum) (void)hipFree(it->accum);
+    wave_free(it->wave);
     if (it->e0) (void)hipEventDestroy(it->e0);
     if (it->e1) (void)hipEventDestroy(it->e1);
     delete it;
@@ -87,7 +92,9 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
     L.spp = p->spp ? p->spp : 1u;
     L.seed = p->seed;
     LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
-    lt_status st = launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+    lt_status st = it->path.strategy == LT_PATH_WAVEFRONT
+                       ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
+                       : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
     if (st != LT_OK) return st;
     LT_HIP(ctx, hipEventRecord(it->e1, ctx->stream));
     it->last.frames = L.spp;

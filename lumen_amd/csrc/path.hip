@@ -54,12 +54,15 @@ LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f
     float bsdf_pdf, bsdf_rev;
     float cos_x = dot(n_s, wi);
     f3 f = eval_bsdf(n_s, wo, mat, side, wi, bsdf_pdf, bsdf_rev, false);
-    if (COUNT) c.shadow++;
-    HitRec h;
-    const bool visible = !trace_bvh<true, COUNT>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, c.nodes, c.tris);
-    if (visible && pdf_light_w > 0.0f) {
+    if (pdf_light_w > 0.0f) {
         const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bsdf_pdf / pdf_light_w);
-        res += mis * f * fabsf(cos_x) * Le / pdf_light_w;
+        const f3 contrib = mis * f * fabsf(cos_x) * Le / pdf_light_w;
+        // the occlusion query only matters for a non-zero contribution
+        if (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) {
+            if (COUNT) c.shadow++;
+            HitRec h;
+            if (!trace_bvh<true, COUNT>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, c.nodes, c.tris)) res += contrib;
+        }
     }
     if ((rec.flags & 7u) == LT_LIGHT_AREA) {
         const float r0 = rnd(seed), r1 = rnd(seed), r2 = rnd(seed);
@@ -209,7 +212,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,
                                  uint64_t* d_counts) {
-    const size_t blocks = (size_t)ctx->num_cus * 4;
+    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.path_blocks_per_cu;
     lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);

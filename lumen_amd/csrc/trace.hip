@@ -62,7 +62,7 @@ lt_status launch_trace(lt_context* ctx, const lt_ray* d_rays, size_t n, void* d_
                        uint64_t* d_counts) {
     if (n == 0) return LT_OK;
     if (n > 0xFFFFFFFFull - 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "too many rays in one call");
-    const size_t blocks = (size_t)ctx->num_cus * 8;
+    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.trace_blocks_per_cu;
     lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);

@@ -64,3 +64,27 @@ def test_running_mean_equals_single_launch(gpu_ctx):
     three = a.output_tex
     a.destroy()
     assert np.array_equal(one, three)
+
+
+@pytest.mark.parametrize("kind,W,H,frames,kw", [
+    ("cornell", 64, 64, 3, {}),
+    ("terrain", 96, 54, 2, dict(terrain_tris=20000, num_instances=100, num_point_lights=8)),
+    ("caustics", 48, 40, 2, {}),
+])
+def test_wavefront_equals_megakernel(gpu_ctx, kind, W, H, frames, kw):
+    """Same per-path random streams and decisions: the two execution forms differ
+    only in the float summation order of the per-vertex terms."""
+    print(f"[gpu] wavefront vs megakernel {kind}", flush=True)
+    sc = la.Scene.procedural(kind, seed=1, **kw)
+    imgs = []
+    for strategy in ("megakernel", "wavefront"):
+        it = la.Path(sc, gpu_ctx, W, H, strategy=strategy)
+        it.init()
+        it.create_accel()
+        it.render(spp=frames)
+        imgs.append(it.output_tex)
+        it.destroy()
+    a, b = imgs
+    assert np.all(np.isfinite(b))
+    d = np.abs(a[..., :3] - b[..., :3])
+    assert np.all(d <= 1e-4 * (1.0 + np.abs(a[..., :3]))), d.max()
