@@ -255,7 +255,9 @@ lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_
 typedef struct lt_tuning {
     uint32_t path_blocks_per_cu;   /* Path megakernel, 128-thread workgroups (default 8) */
     uint32_t trace_blocks_per_cu;  /* raw queries, 128-thread workgroups (default 8)     */
-    uint32_t reserved[14];
+    uint32_t refill_min;           /* wavefront trace stages: idle lanes that trigger a
+                                      refill from the queue, 1..64 (default 32)         */
+    uint32_t reserved[13];
 } lt_tuning;
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);

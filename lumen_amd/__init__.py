@@ -121,7 +121,8 @@ class Context:
     def get_tuning(self) -> dict:
         t = _abi.lt_tuning()
         check(lib().lt_get_tuning(self._h, C.byref(t)), self._h, "lt_get_tuning")
-        return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+        return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu,
+                "refill_min": t.refill_min}
 
     def upload(self, scene: Scene) -> None:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")

@@ -110,7 +110,7 @@ class lt_procedural_params(C.Structure):
 
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("reserved", C.c_uint32 * 14)]
+                ("refill_min", C.c_uint32), ("reserved", C.c_uint32 * 13)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

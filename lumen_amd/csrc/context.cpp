@@ -37,6 +37,7 @@ SceneView make_scene_view(const lt_context* ctx) {
     v.tris = ctx->bvh.tris;
     v.num_lights = s.nlights;
     v.num_nodes = ctx->bvh.built ? ctx->bvh.n_nodes : 0;
+    v.refill_min = ctx->tuning.refill_min;
     v.total_light_triangle_cnt = s.total_light_triangle_cnt;
     v.dir_light_idx = s.dir_light_idx;
     v.total_light_area = s.total_light_area;
@@ -294,6 +295,8 @@ lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "blocks per CU must be <= 64");
     if (t->path_blocks_per_cu) ctx->tuning.path_blocks_per_cu = t->path_blocks_per_cu;
     if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
+    if (t->refill_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "refill_min must be in 1..64");
+    if (t->refill_min) ctx->tuning.refill_min = t->refill_min;
     return LT_OK;
 }
 

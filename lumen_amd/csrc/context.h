@@ -28,6 +28,7 @@ struct SceneView {
     const ltb::TriRec* tris;
     uint32_t num_lights;
     uint32_t num_nodes;
+    uint32_t refill_min;  // wavefront trace stages (lt_tuning)
     uint32_t total_light_triangle_cnt;
     int32_t dir_light_idx;
     float total_light_area;
@@ -80,7 +81,7 @@ struct lt_context {
     size_t hit_buf_bytes = 0;
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
-    lt_tuning tuning{8u, 8u, {}};
+    lt_tuning tuning{8u, 8u, 32u, {}};
 };
 
 // helpers shared by the .cpp / .hip translation units

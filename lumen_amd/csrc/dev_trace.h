@@ -96,8 +96,10 @@ struct HitRec {
     uint32_t gid, prim, inst;  // gid = 0xFFFFFFFF on a miss
 };
 
-// gamma(3) widening of PBRT's robust slab test
-constexpr float kBoxWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-8f) / (1.0f - 3.0f * 5.9604645e-8f);
+// Relative bound (2^-21) on the rounding of a child slab distance q * a + b:
+// a = 2^e * inv and b = (p - o) * inv each carry one rounding, inv one more
+// and the fused multiply-add one, i.e. |error| < 4 eps (65535 |a| + |b|).
+constexpr float kSlabErr = 4.76837158e-7f;
 
 struct TraceStack {
     uint32_t* lds;    // this thread's column: entry i at lds[i * kTraceBlock]
@@ -116,21 +118,45 @@ LT_DEV uint32_t stk_pop(const TraceStack& s, uint32_t& sp) {
     return sp < (uint32_t)ltb::kLdsStack ? s.lds[sp * kTraceBlock] : s.spill[sp - ltb::kLdsStack];
 }
 
-// Traverses the BVH8.  ANY: returns at the first hit in [tmin, tmax].
-// Counters (nodes popped, triangles tested) are only updated when COUNT.
+// The traversal of one query as a state machine, so that a lane can take a
+// new query as soon as its current one finishes (trace_step returns true).
+struct TraceState {
+    f3 o, d;
+    RayPre rp;
+    float tmin, tmax, tcur;
+    HitRec best;
+    uint32_t sp;
+};
+
+LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float tmin, float tmax, bool has_nodes) {
+    s.o = o;
+    s.d = d;
+    s.tmin = tmin;
+    s.tmax = tmax;
+    s.tcur = tmax;
+    s.best.gid = 0xFFFFFFFFu;
+    s.best.t = tmax;
+    s.best.prim = s.best.inst = 0xFFFFFFFFu;
+    s.best.u = s.best.v = 0.0f;
+    s.rp = prep_ray(d);
+    s.sp = 0;
+    if (has_nodes) stk_push(stk, s.sp, 0u);
+}
+
+// Processes one BVH8 node (its children's boxes, then its leaf triangles,
+// then pushes the hit internal children, nearest last).  ANY: finishes at the
+// first hit in [tmin, tmax].  Counters are only updated when COUNT.
 template <bool ANY, bool COUNT>
-LT_DEV bool trace_bvh(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, HitRec& best, const TraceStack& stk,
-                      uint32_t& n_nodes, uint32_t& n_tris) {
-    best.gid = 0xFFFFFFFFu;
-    best.t = tmax;
-    best.prim = best.inst = 0xFFFFFFFFu;
-    best.u = best.v = 0.0f;
-    if (sv.num_nodes == 0) return false;
-    const RayPre rp = prep_ray(d);
-    float tcur = tmax;
-    uint32_t sp = 0;
-    stk_push(stk, sp, 0u);
-    while (sp > 0) {
+LT_DEV bool trace_step(const SceneView& sv, TraceState& s, const TraceStack& stk, uint32_t& n_nodes,
+                       uint32_t& n_tris) {
+    if (s.sp == 0) return true;
+    const f3 o = s.o;
+    const RayPre& rp = s.rp;
+    const float tmin = s.tmin, tmax = s.tmax;
+    float& tcur = s.tcur;
+    HitRec& best = s.best;
+    uint32_t& sp = s.sp;
+    {
         const uint32_t ni = stk_pop(stk, sp);
         if (COUNT) n_nodes++;
         const ltb::Node8* np = sv.nodes + ni;
@@ -142,28 +168,37 @@ LT_DEV bool trace_bvh(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, H
         const uint4 q3 = *((const uint4*)np + 5);
         const uint4 q4 = *((const uint4*)np + 6);
         const uint4 q5 = *((const uint4*)np + 7);
-        const float px = w0.x, py = w0.y, pz = w0.z;
         const uint32_t ew = __float_as_uint(w0.w);  // e.x, e.y, e.z, imask
         const int ex = (int)(int8_t)(ew & 0xFF), ey = (int)(int8_t)((ew >> 8) & 0xFF),
                   ez = (int)(int8_t)((ew >> 16) & 0xFF);
         const uint32_t imask = ew >> 24;
-        const float sx = __int_as_float((ex + 127) << 23), sy = __int_as_float((ey + 127) << 23),
-                    sz = __int_as_float((ez + 127) << 23);
         const uint32_t child_base = w1.x, tri_base = w1.y;
         const uint32_t meta_lo = w1.z, meta_hi = w1.w;
-        const uint32_t qlx[4] = {q0.x, q0.y, q0.z, q0.w}, qly[4] = {q1.x, q1.y, q1.z, q1.w},
-                       qlz[4] = {q2.x, q2.y, q2.z, q2.w};
-        const uint32_t qhx[4] = {q3.x, q3.y, q3.z, q3.w}, qhy[4] = {q4.x, q4.y, q4.z, q4.w},
-                       qhz[4] = {q5.x, q5.y, q5.z, q5.w};
+        // Slab distances of the children: t = q * a + b per axis with
+        // a = 2^e / d and b = (p - o) / d.  E bounds the rounding of that
+        // evaluation against the exact (p + q 2^e - o) / d, so
+        // [t_near - E, t_far + E] contains every child's true slab interval.
+        const float ax = __int_as_float((ex + 127) << 23) * rp.inv.x;
+        const float ay = __int_as_float((ey + 127) << 23) * rp.inv.y;
+        const float az = __int_as_float((ez + 127) << 23) * rp.inv.z;
+        const float bx = (w0.x - o.x) * rp.inv.x, by = (w0.y - o.y) * rp.inv.y, bz = (w0.z - o.z) * rp.inv.z;
+        const float Ex = kSlabErr * fmaf(65535.0f, fabsf(ax), fabsf(bx));
+        const float Ey = kSlabErr * fmaf(65535.0f, fabsf(ay), fabsf(by));
+        const float Ez = kSlabErr * fmaf(65535.0f, fabsf(az), fabsf(bz));
+        const float bnx = bx - Ex, bfx = bx + Ex, bny = by - Ey, bfy = by + Ey, bnz = bz - Ez, bfz = bz + Ez;
+        // near / far planes by the ray octant
+        const bool negx = rp.inv.x < 0.0f, negy = rp.inv.y < 0.0f, negz = rp.inv.z < 0.0f;
+        const uint4 qnx = negx ? q3 : q0, qfx = negx ? q0 : q3;
+        const uint4 qny = negy ? q4 : q1, qfy = negy ? q1 : q4;
+        const uint4 qnz = negz ? q5 : q2, qfz = negz ? q2 : q5;
         // The watertight t of a triangle carries an absolute error of a few ulp
         // of the largest sheared z of its vertices, i.e. of the farthest t of
         // the enclosing box along the dominant axis kz.  Children are kept if
         // they reach [tmin - margin, tcur + margin], so a triangle whose computed
         // t undercuts the current best by that error is still tested.
-        const float pkz = rp.kz == 0 ? px : (rp.kz == 1 ? py : pz);
-        const float skz = rp.kz == 0 ? sx : (rp.kz == 1 ? sy : sz);
-        const float okz = sel(o, rp.kz), ikz = sel(rp.inv, rp.kz);
-        const float far_kz = fmaxf(fabsf((pkz - okz) * ikz), fabsf((pkz + 65535.0f * skz - okz) * ikz));
+        const float akz = rp.kz == 0 ? ax : (rp.kz == 1 ? ay : az);
+        const float bkz = rp.kz == 0 ? bx : (rp.kz == 1 ? by : bz);
+        const float far_kz = fmaxf(fabsf(bkz), fabsf(fmaf(65535.0f, akz, bkz)));
         const float margin = far_kz * (1.0f / 65536.0f);
         const float lo_lim = tmin - margin;
         const float hi_lim = tcur + margin;
@@ -172,27 +207,19 @@ LT_DEV bool trace_bvh(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, H
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int sh = (c & 1) * 16;
-            const float lx = px + (float)((qlx[c >> 1] >> sh) & 0xFFFF) * sx;
-            const float ly = py + (float)((qly[c >> 1] >> sh) & 0xFFFF) * sy;
-            const float lz = pz + (float)((qlz[c >> 1] >> sh) & 0xFFFF) * sz;
-            const float hx = px + (float)((qhx[c >> 1] >> sh) & 0xFFFF) * sx;
-            const float hy = py + (float)((qhy[c >> 1] >> sh) & 0xFFFF) * sy;
-            const float hz = pz + (float)((qhz[c >> 1] >> sh) & 0xFFFF) * sz;
-            float ta = (lx - o.x) * rp.inv.x, tb = (hx - o.x) * rp.inv.x;
-            float t0 = fminf(ta, tb), t1 = fmaxf(ta, tb);
-            ta = (ly - o.y) * rp.inv.y;
-            tb = (hy - o.y) * rp.inv.y;
-            t0 = fmaxf(t0, fminf(ta, tb));
-            t1 = fminf(t1, fmaxf(ta, tb));
-            ta = (lz - o.z) * rp.inv.z;
-            tb = (hz - o.z) * rp.inv.z;
-            t0 = fmaxf(t0, fminf(ta, tb));
-            t1 = fminf(t1, fmaxf(ta, tb));
-            t0 = t0 > 0.0f ? t0 * (1.0f / kBoxWiden) : t0 * kBoxWiden;
-            t1 = t1 > 0.0f ? t1 * kBoxWiden : t1 * (1.0f / kBoxWiden);
+            const int wd = c >> 1;
+            auto lane16 = [&](const uint4& q) {
+                const uint32_t w = wd == 0 ? q.x : (wd == 1 ? q.y : (wd == 2 ? q.z : q.w));
+                return (float)((w >> sh) & 0xFFFFu);
+            };
+            float t0 = fmaxf(fmaxf(fmaf(lane16(qnx), ax, bnx), fmaf(lane16(qny), ay, bny)), fmaf(lane16(qnz), az, bnz));
+            float t1 = fminf(fminf(fmaf(lane16(qfx), ax, bfx), fmaf(lane16(qfy), ay, bfy)), fmaf(lane16(qfz), az, bfz));
             t0 = fmaxf(t0, lo_lim);
             t1 = fminf(t1, hi_lim);
-            const bool h = (t0 <= t1) && (lx <= hx);
+            const uint32_t wlo = wd == 0 ? q0.x : (wd == 1 ? q0.y : (wd == 2 ? q0.z : q0.w));
+            const uint32_t whi = wd == 0 ? q3.x : (wd == 1 ? q3.y : (wd == 2 ? q3.z : q3.w));
+            const bool valid = ((wlo >> sh) & 0xFFFFu) <= ((whi >> sh) & 0xFFFFu);  // empty slots: lo > hi
+            const bool h = (t0 <= t1) && valid;
             tn_c[c] = t0;
             const uint32_t meta = ((c < 4 ? meta_lo : meta_hi) >> ((c & 3) * 8)) & 0xFF;
             const bool internal = (imask >> c) & 1u;
@@ -255,7 +282,65 @@ LT_DEV bool trace_bvh(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, H
             stk_push(stk, sp, child_base + meta);
         }
     }
-    return best.gid != 0xFFFFFFFFu;
+    return sp == 0;
+}
+
+// Streams the queries [0, n_items) of a queue through one wave: every lane
+// steps its own query one node at a time, and lanes whose query finished
+// take new ones (one atomic per refill, refills once sv.refill_min lanes are
+// idle), so the wave stays full however uneven the traversal lengths are.
+// load(item, o, d, tmin, tmax) returns a tag handed to finish(tag, state).
+template <bool ANY, bool COUNT, class Load, class Finish>
+LT_DEV void trace_stream(const SceneView& sv, const TraceStack& stk, uint32_t n_items, uint32_t* work, Load load,
+                         Finish finish, uint32_t& n_nodes, uint32_t& n_tris) {
+    const uint32_t lane = threadIdx.x & 63u;
+    TraceState s;
+    uint32_t tag = 0;
+    bool has = false;
+    bool exhausted = false;  // wave-uniform
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long idle = __ballot(!has);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= sv.refill_min || n_idle == 64u) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, n_idle);
+                base = __shfl(base, leader);
+                if (base + n_idle >= n_items) exhausted = true;
+                if (!has) {
+                    const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (mine < n_items) {
+                        f3 o, d;
+                        float tmin, tmax;
+                        tag = load(mine, o, d, tmin, tmax);
+                        trace_init(s, stk, o, d, tmin, tmax, sv.num_nodes != 0);
+                        has = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && trace_step<ANY, COUNT>(sv, s, stk, n_nodes, n_tris)) {
+            finish(tag, s);
+            has = false;
+        }
+    }
+}
+
+// Whole query on one lane.
+template <bool ANY, bool COUNT>
+LT_DEV bool trace_bvh(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, HitRec& best, const TraceStack& stk,
+                      uint32_t& n_nodes, uint32_t& n_tris) {
+    TraceState s;
+    trace_init(s, stk, o, d, tmin, tmax, sv.num_nodes != 0);
+    while (!trace_step<ANY, COUNT>(sv, s, stk, n_nodes, n_tris)) {
+    }
+    best = s.best;
+    return s.best.gid != 0xFFFFFFFFu;
 }
 
 }  // namespace ltd

@@ -97,17 +97,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_wf_extend(SceneView sv, WaveSta
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t* q = S.queue[depth & 1];
     uint32_t n_nodes = 0, n_tris = 0;
-    WAVE_LOOP(ctl->q_len[depth], &ctl->work[depth][0])
-    if (active) {
+    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
         const uint32_t p = q[item];
-        const float4 o = S.ray_o[p], d = S.ray_d[p];
-        HitRec h;
-        const bool hit =
-            trace_bvh<false, COUNT>(sv, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z), PATH_TMIN, PATH_TMAX, h, stk, n_nodes, n_tris);
-        S.hit[p] = make_uint4(hit ? h.prim : 0xFFFFFFFFu, hit ? h.inst : 0xFFFFFFFFu, __float_as_uint(h.u),
-                              __float_as_uint(h.v));
-    }
-    WAVE_LOOP_END
+        const float4 ro = S.ray_o[p], rd = S.ray_d[p];
+        o = mk3(ro.x, ro.y, ro.z);
+        d = mk3(rd.x, rd.y, rd.z);
+        tmin = PATH_TMIN;
+        tmax = PATH_TMAX;
+        return p;
+    };
+    auto finish = [&](uint32_t p, const TraceState& s) {
+        const bool hit = s.best.gid != 0xFFFFFFFFu;
+        S.hit[p] = make_uint4(hit ? s.best.prim : 0xFFFFFFFFu, hit ? s.best.inst : 0xFFFFFFFFu,
+                              __float_as_uint(s.best.u), __float_as_uint(s.best.v));
+    };
+    trace_stream<false, COUNT>(sv, stk, ctl->q_len[depth], &ctl->work[depth][0], load, finish, n_nodes, n_tris);
     if (COUNT) {
         unsigned long long a = n_nodes, b = n_tris;
         for (int o = 32; o > 0; o >>= 1) {
@@ -274,23 +278,26 @@ __global__ __launch_bounds__(kTraceBlock) void k_wf_shadow(SceneView sv, WaveSta
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    WAVE_LOOP(ctl->sh_len[depth], &ctl->work[depth][2])
-    if (active) {
-        const ShadowRec r = S.shadow[item];
-        HitRec h;
+    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4 ro = S.shadow[item].o, rd = S.shadow[item].d;
+        o = mk3(ro.x, ro.y, ro.z);
+        d = mk3(rd.x, rd.y, rd.z);
+        tmin = 0.0f;
+        tmax = ro.w;
         if (COUNT) n_rays++;
-        const bool occ =
-            trace_bvh<true, COUNT>(sv, mk3(r.o.x, r.o.y, r.o.z), mk3(r.d.x, r.d.y, r.d.z), 0.0f, r.o.w, h, stk, n_nodes, n_tris);
-        if (!occ) {
-            const uint32_t p = __float_as_uint(r.d.w);
-            float4 c = S.col[p];
-            c.x += r.c.x;
-            c.y += r.c.y;
-            c.z += r.c.z;
-            S.col[p] = c;
-        }
-    }
-    WAVE_LOOP_END
+        return item;
+    };
+    auto finish = [&](uint32_t item, const TraceState& s) {
+        if (s.best.gid != 0xFFFFFFFFu) return;  // occluded
+        const float4 rd = S.shadow[item].d, rc = S.shadow[item].c;
+        const uint32_t p = __float_as_uint(rd.w);
+        float4 c = S.col[p];
+        c.x += rc.x;
+        c.y += rc.y;
+        c.z += rc.z;
+        S.col[p] = c;
+    };
+    trace_stream<true, COUNT>(sv, stk, ctl->sh_len[depth], &ctl->work[depth][2], load, finish, n_nodes, n_tris);
     if (COUNT) {
         unsigned long long a = n_nodes, b = n_tris, s = n_rays;
         for (int o = 32; o > 0; o >>= 1) {
@@ -317,14 +324,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_wf_mis(SceneView sv, WaveState 
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    WAVE_LOOP(ctl->mis_len[depth], &ctl->work[depth][3])
-    if (active) {
-        const MisRec r = S.mis[item];
+    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4 ro = S.mis[item].o, rd = S.mis[item].d;
+        o = mk3(ro.x, ro.y, ro.z);
+        d = mk3(rd.x, rd.y, rd.z);
+        tmin = PATH_TMIN;
+        tmax = PATH_TMAX;
         if (COUNT) n_rays++;
-        HitRec h;
+        return item;
+    };
+    auto finish = [&](uint32_t item, const TraceState& s) {
+        const MisRec r = S.mis[item];
         const f3 wi = mk3(r.d.x, r.d.y, r.d.z);
-        const bool hit = trace_bvh<false, COUNT>(sv, mk3(r.o.x, r.o.y, r.o.z), wi, PATH_TMIN, PATH_TMAX, h, stk,
-                                                 n_nodes, n_tris);
+        const bool hit = s.best.gid != 0xFFFFFFFFu;
+        const HitRec& h = s.best;
         uint32_t tri = r.ids.z, inst = r.ids.w;
         f3 ppos = mk3(r.vpos.x, r.vpos.y, r.vpos.z), pns = mk3(r.sns.x, r.sns.y, r.sns.z);
         if (hit) {
@@ -347,8 +360,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_wf_mis(SceneView sv, WaveState 
             c.z += r.w.z * mis;
             S.col[p] = c;
         }
-    }
-    WAVE_LOOP_END
+    };
+    trace_stream<false, COUNT>(sv, stk, ctl->mis_len[depth], &ctl->work[depth][3], load, finish, n_nodes, n_tris);
     if (COUNT) {
         unsigned long long a = n_nodes, b = n_tris, s = n_rays;
         for (int o = 32; o > 0; o >>= 1) {

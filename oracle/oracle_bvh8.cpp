@@ -26,7 +26,7 @@ struct TriRech {
 };
 static_assert(sizeof(TriRech) == 48, "tri layout");
 
-static const float kWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-8f) / (1.0f - 3.0f * 5.9604645e-8f);
+static const float kSlabErr = 4.76837158e-7f;  // 2^-21
 
 }  // namespace ora
 
@@ -54,42 +54,39 @@ extern "C" void ora_bvh8_trace(const void* nodes_v, uint32_t n_nodes, const void
             const Node8h& nd = nodes[stack.back()];
             stack.pop_back();
             nn++;
-            float s[3];
+            // child slab distances t = q * a + b, a = 2^e / d, b = (p - o) / d,
+            // widened by the rounding bound E = 2^-21 (65535 |a| + |b|)
+            float a[3], b[3], bn[3], bf[3];
             for (int k = 0; k < 3; k++) {
-                uint32_t b = (uint32_t)(nd.e[k] + 127) << 23;
-                memcpy(&s[k], &b, 4);
+                uint32_t bits = (uint32_t)(nd.e[k] + 127) << 23;
+                float s;
+                memcpy(&s, &bits, 4);
+                a[k] = s * inv[k];
+                b[k] = (nd.p[k] - o[k]) * inv[k];
+                float E = kSlabErr * fmaf(65535.0f, fabsf(a[k]), fabsf(b[k]));
+                bn[k] = b[k] - E;
+                bf[k] = b[k] + E;
             }
             // cull margin: 2^-16 of the node's far t along the dominant axis kz
             // (bounds the error of a contained triangle's watertight t)
-            float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-            int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-            float far_kz = fmaxf(fabsf((nd.p[kz] - o[kz]) * inv[kz]),
-                                 fabsf((nd.p[kz] + 65535.0f * s[kz] - o[kz]) * inv[kz]));
+            float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+            int kz = (adx > ady) ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
+            float far_kz = fmaxf(fabsf(b[kz]), fabsf(fmaf(65535.0f, a[kz], b[kz])));
             float margin = far_kz * (1.0f / 65536.0f);
             uint32_t hit_int = 0, hit_leaf = 0;
             float tn[8];
             for (int c = 0; c < 8; c++) {
-                float lo[3], hi[3];
+                float qn[3], qf[3];
                 for (int k = 0; k < 3; k++) {
-                    lo[k] = nd.p[k] + (float)nd.qlo[k][c] * s[k];
-                    hi[k] = nd.p[k] + (float)nd.qhi[k][c] * s[k];
+                    bool neg = inv[k] < 0.0f;  // ray octant picks the near plane
+                    qn[k] = (float)(neg ? nd.qhi[k][c] : nd.qlo[k][c]);
+                    qf[k] = (float)(neg ? nd.qlo[k][c] : nd.qhi[k][c]);
                 }
-                float t0 = -INFINITY, t1 = INFINITY;
-                for (int k = 0; k < 3; k++) {
-                    float ta = (lo[k] - o[k]) * inv[k], tb = (hi[k] - o[k]) * inv[k];
-                    if (k == 0) {
-                        t0 = fminf(ta, tb);
-                        t1 = fmaxf(ta, tb);
-                    } else {
-                        t0 = fmaxf(t0, fminf(ta, tb));
-                        t1 = fminf(t1, fmaxf(ta, tb));
-                    }
-                }
-                t0 = t0 > 0.0f ? t0 * (1.0f / kWiden) : t0 * kWiden;
-                t1 = t1 > 0.0f ? t1 * kWiden : t1 * (1.0f / kWiden);
+                float t0 = fmaxf(fmaxf(fmaf(qn[0], a[0], bn[0]), fmaf(qn[1], a[1], bn[1])), fmaf(qn[2], a[2], bn[2]));
+                float t1 = fminf(fminf(fmaf(qf[0], a[0], bf[0]), fmaf(qf[1], a[1], bf[1])), fmaf(qf[2], a[2], bf[2]));
                 t0 = fmaxf(t0, tmin - margin);
                 t1 = fminf(t1, tcur + margin);
-                bool h = (t0 <= t1) && (lo[0] <= hi[0]);
+                bool h = (t0 <= t1) && (nd.qlo[0][c] <= nd.qhi[0][c]);
                 tn[c] = t0;
                 bool internal = (nd.imask >> c) & 1u;
                 if (h && internal) hit_int |= 1u << c;
