@@ -30,6 +30,12 @@ class PathParams(C.Structure):
                 ("x1", C.c_uint32), ("y1", C.c_uint32)]
 
 
+class BdptParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
+                ("frame_start", C.c_uint32), ("frames", C.c_uint32), ("seed", C.c_uint32), ("power", C.c_uint32),
+                ("x0", C.c_uint32), ("y0", C.c_uint32), ("x1", C.c_uint32), ("y1", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -68,6 +74,10 @@ def lib() -> C.CDLL:
         L.ora_render_path.argtypes = [P, P, P, C.POINTER(PathParams), P, C.c_int, C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]
         L.ora_render_path.restype = C.c_double
+        L.ora_render_bdpt.argtypes = [P, P, P, C.POINTER(BdptParams), P, P, C.c_int, C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+        L.ora_render_bdpt.restype = C.c_double
+        L.ora_sample_light_Le.argtypes = [P, P, P]
         L.ora_hardware_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -141,6 +151,14 @@ def sample_light_Li(desc, rands4, p):
     return out
 
 
+def sample_light_Le(desc, rands6):
+    """[Le3, pos3, wi3, n3, cos_from_light, pdf_pos_a, pdf_dir_w, flags]"""
+    rands6 = _f3(rands6)
+    out = np.zeros(16, np.float32)
+    lib().ora_sample_light_Le(C.byref(desc), rands6.ctypes.data, out.ctypes.data)
+    return out
+
+
 class Accel:
     """Host BVH2 over the scene's triangles (+ brute force)."""
 
@@ -186,6 +204,24 @@ class Accel:
         cl, sh = C.c_uint64(), C.c_uint64()
         secs = lib().ora_render_path(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), accum.ctypes.data,
                                      threads or lib().ora_hardware_threads(), C.byref(cl), C.byref(sh))
+        return accum, secs, cl.value, sh.value
+
+    def render_bdpt(self, camera, width, height, max_depth, frame_start=0, frames=1, seed=0, power=False,
+                    window=None, accum=None, splat=None, threads=0):
+        """Runs the reference BDPT integrator on the CPU; returns (image, seconds, closest, shadow)."""
+        p = BdptParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.frame_start, p.frames, p.seed, p.power = frame_start, frames, seed, int(power)
+        if window:
+            p.x0, p.y0, p.x1, p.y1 = window
+        if accum is None:
+            accum = np.zeros((height, width, 4), np.float32)
+        if splat is None:
+            splat = np.zeros((height, width, 3), np.float32)
+        cl, sh = C.c_uint64(), C.c_uint64()
+        secs = lib().ora_render_bdpt(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), accum.ctypes.data,
+                                     splat.ctypes.data, threads or lib().ora_hardware_threads(), C.byref(cl),
+                                     C.byref(sh))
         return accum, secs, cl.value, sh.value
 
     def __del__(self):

@@ -118,7 +118,12 @@ def camera(pos, look_dir, fov_deg=40.0, aspect=1.0):
     """A pinhole camera in the SceneUBO convention the kernels read
     (sample_camera, commons.glsl:30-33): inv_view maps view to world, the
     view looks down -z; inv_projection maps NDC (x, y, 1, 1) to a view-space
-    point.  Built here from first principles (orthonormal frame + tan(fov/2))."""
+    point.  Built here from first principles (orthonormal frame + tan(fov/2)).
+    The inverse projection also maps NDC (x, y, 0, 1) to the pixel's point on
+    the z = -1 plane after the divide by w (the camera-area computation of
+    bdpt.rgen:48-51), and view / projection are its exact inverses, so the
+    light-tracing raster projection (bdpt_commons.glsl:519-523) lands on the
+    pixel whose camera ray points the same way."""
     f = np.asarray(look_dir, np.float64)
     f /= np.linalg.norm(f)
     up = np.array([0, 1, 0], np.float64) if abs(f[1]) < 0.99 else np.array([0, 0, 1], np.float64)
@@ -128,9 +133,18 @@ def camera(pos, look_dir, fov_deg=40.0, aspect=1.0):
     inv_view = np.eye(4)
     inv_view[:3, 0], inv_view[:3, 1], inv_view[:3, 2], inv_view[:3, 3] = r, u, -f, pos
     t = np.tan(np.radians(fov_deg) / 2)
+    alpha = -0.5  # any non-zero value: only the w row at z = 1 depends on it
     inv_proj = np.diag([t * aspect, t, 0.0, 1.0])
-    inv_proj[2, 3] = -1.0  # (x, y, 1, 1) -> (x tan*a, y tan, -1, 1)
+    inv_proj[2, 3] = -1.0  # (x, y, z, 1) -> (x tan*a, y tan, -1, alpha z + 1)
+    inv_proj[3, 2] = alpha
+    proj = np.zeros((4, 4))
+    proj[0, 0], proj[1, 1] = 1.0 / (t * aspect), 1.0 / t
+    proj[2, 2] = proj[2, 3] = 1.0 / alpha
+    proj[3, 2] = -1.0
+    view = np.linalg.inv(inv_view)
     cam = _abi.lt_camera()
-    cam.inv_view[:] = list(inv_view.T.reshape(-1).astype(np.float32))  # column-major
+    cam.view[:] = list(view.T.reshape(-1).astype(np.float32))  # column-major
+    cam.projection[:] = list(proj.T.reshape(-1).astype(np.float32))
+    cam.inv_view[:] = list(inv_view.T.reshape(-1).astype(np.float32))
     cam.inv_projection[:] = list(inv_proj.T.reshape(-1).astype(np.float32))
     return cam
