@@ -257,7 +257,9 @@ typedef struct lt_tuning {
     uint32_t trace_blocks_per_cu;  /* raw queries, 128-thread workgroups (default 8)     */
     uint32_t refill_min;           /* wavefront trace stages: idle lanes that trigger a
                                       refill from the queue, 1..64 (default 32)         */
-    uint32_t reserved[13];
+    uint32_t bdpt_blocks_per_cu;   /* BDPT walk and connection kernels, 128-thread
+                                      workgroups (default 4)                            */
+    uint32_t reserved[12];
 } lt_tuning;
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);
@@ -302,6 +304,22 @@ typedef struct lt_path_config {
     uint32_t reserved[6];
 } lt_path_config;
 
+/* MIS heuristics of the BDPT integrator. */
+#define LT_MIS_BALANCE 0u /* calc_mis_weight of the reference (bdpt_commons.glsl:428-472) */
+#define LT_MIS_POWER 1u   /* power heuristic, beta = 2                                     */
+
+/* BDPT integrator config: SceneConfig/BDPTConfig (src/RayTracer/SceneConfig.h:27-29)
+ * plus the PCBDPT fields set by BDPT::render (src/RayTracer/BDPT.cpp:55-64).
+ * Camera rays go through pixel centres (bdpt.rgen:41-42); light and camera
+ * subpaths hold up to max_depth + 1 vertices each. */
+typedef struct lt_bdpt_config {
+    uint32_t width;
+    uint32_t height;
+    int32_t max_depth;   /* path_length, 1..16 (default 6) */
+    uint32_t mis;        /* LT_MIS_BALANCE or LT_MIS_POWER */
+    uint32_t reserved[8];
+} lt_bdpt_config;
+
 /* Per-frame parameters (SceneUBO + push-constant frame_num). */
 typedef struct lt_frame_params {
     lt_camera camera;
@@ -323,8 +341,8 @@ typedef struct lt_render_stats {
 
 /* Replaces RayTracer::create_integrator + Integrator::init
  * (src/RayTracer/RayTracer.cpp:244-282, Integrator.cpp:8-50).  cfg points to
- * the config struct of `kind` (lt_path_config for LT_INTEGRATOR_PATH) and
- * cfg_size must equal its size. */
+ * the config struct of `kind` (lt_path_config for LT_INTEGRATOR_PATH,
+ * lt_bdpt_config for LT_INTEGRATOR_BDPT) and cfg_size must equal its size. */
 lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const void* cfg, size_t cfg_size,
                                lt_integrator** out);
 lt_status lt_integrator_destroy(lt_integrator* it);

@@ -17,7 +17,7 @@ from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRA
                    LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS, LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL,
                    LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
 
-__all__ = ["Scene", "Context", "Integrator", "Path", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
+__all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
            "lib", "make_rays"]
 
 _KINDS = {"cornell": LT_SCENE_CORNELL, "terrain": LT_SCENE_TERRAIN, "caustics": LT_SCENE_CAUSTICS,
@@ -122,7 +122,7 @@ class Context:
         t = _abi.lt_tuning()
         check(lib().lt_get_tuning(self._h, C.byref(t)), self._h, "lt_get_tuning")
         return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu,
-                "refill_min": t.refill_min}
+                "refill_min": t.refill_min, "bdpt_blocks_per_cu": t.bdpt_blocks_per_cu}
 
     def upload(self, scene: Scene) -> None:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
@@ -338,6 +338,28 @@ class Path(Integrator):
         return c
 
 
+class BDPT(Integrator):
+    """Bidirectional path tracing (src/RayTracer/BDPT.{h,cpp}): light and camera
+    subpaths of up to path_length + 1 vertices, every (s, t) connection,
+    light-tracing splats, MIS with the reference's balance heuristic
+    (mis="balance") or the power heuristic (mis="power")."""
+
+    kind = LT_INTEGRATOR_BDPT
+
+    def __init__(self, lumen_scene, ctx, width=1920, height=1080, path_length=None, mis="balance"):
+        super().__init__(lumen_scene, ctx, width, height)
+        icfg = lumen_scene.integrator_cfg()
+        self.path_length = path_length if path_length is not None else (icfg.path_length or 6)
+        self.mis = {"balance": _abi.LT_MIS_BALANCE, "power": _abi.LT_MIS_POWER}[mis]
+
+    def _config(self):
+        c = _abi.lt_bdpt_config()
+        c.width, c.height = self.width, self.height
+        c.max_depth = self.path_length
+        c.mis = self.mis
+        return c
+
+
 def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 1920, height: int = 1080,
                       **kw) -> Integrator:
     """RayTracer::create_integrator (RayTracer.cpp:244-282), keyed by the
@@ -345,6 +367,8 @@ def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 
     n = name.lower()
     if n == "path":
         return Path(lumen_scene, ctx, width, height, **kw)
+    if n == "bdpt":
+        return BDPT(lumen_scene, ctx, width, height, **kw)
     raise LtError(6, f"integrator '{name}' is not built")
 
 

@@ -28,6 +28,7 @@ LT_LIGHT_FINITE, LT_LIGHT_DELTA = 1 << 4, 1 << 5
 LT_PTR_DEVICE = 1
 LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM = 0, 1, 2
 LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT = 0, 1
+LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
 
@@ -92,6 +93,11 @@ class lt_path_config(C.Structure):
                 ("reserved", C.c_uint32 * 6)]
 
 
+class lt_bdpt_config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("mis", C.c_uint32),
+                ("reserved", C.c_uint32 * 8)]
+
+
 class lt_frame_params(C.Structure):
     _fields_ = [("camera", lt_camera), ("frame_num", C.c_uint32), ("seed", C.c_uint32), ("spp", C.c_uint32),
                 ("reserved", C.c_uint32 * 5)]
@@ -110,7 +116,7 @@ class lt_procedural_params(C.Structure):
 
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("refill_min", C.c_uint32), ("reserved", C.c_uint32 * 13)]
+                ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 12)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

@@ -81,7 +81,7 @@ struct lt_context {
     size_t hit_buf_bytes = 0;
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
-    lt_tuning tuning{8u, 8u, 32u, {}};
+    lt_tuning tuning{8u, 8u, 32u, 4u, {}};
 };
 
 // helpers shared by the .cpp / .hip translation units

@@ -14,6 +14,7 @@ struct Payload {
     f3 n_g, n_s, pos;
     f2 uv;
     uint32_t material_idx, triangle_idx, instance_idx;
+    float area;  // world-space triangle area
     bool valid;
 };
 
@@ -30,7 +31,9 @@ LT_DEV void fill_payload(const SceneView& sv, uint32_t inst, uint32_t prim, floa
     const f3 nrm = normalize(ld3(a.normal) * b0 + ld3(b.normal) * u + ld3(c.normal) * v);
     p.n_s = normalize(nrm);
     p.uv = {a.uv0[0] * b0 + b.uv0[0] * u + c.uv0[0] * v, a.uv0[1] * b0 + b.uv0[1] * u + c.uv0[1] * v};
-    p.n_g = normalize(cross(v2 - v0, v1 - v0));
+    const f3 cg = cross(v2 - v0, v1 - v0);
+    p.n_g = normalize(cg);
+    p.area = 0.5f * length(cg);
     p.material_idx = pi.material_index;
     p.triangle_idx = prim;
     p.instance_idx = inst;
@@ -120,15 +123,17 @@ LT_DEV float spot_falloff(float cos_from_light, float cos_width, float cos_fallo
     return (d * d) * (d * d);
 }
 
-// commons.glsl:224-300 (+ LT_LIGHT_POINT).  rp = rand4.
+// commons.glsl:224-300 (+ LT_LIGHT_POINT).  rp = rand4; n and pos are the
+// sampled light point's normal and position (commons.glsl:302-307).
 LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,
-                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec) {
+                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec, f3& n, f3& pos) {
     rec.light_idx = (uint32_t)(rp.x * (float)sv.num_lights);
     const lt_light& L = sv.lights[rec.light_idx];
     const uint32_t flags = L.light_flags;
     const uint32_t lt = flags & 7u;
     f3 Le = mk3(0.0f);
     rec.flags = flags;
+    n = pos = mk3(0.0f);
     pdf_pos_w = 0.0f;
     wi = mk3(0.0f);
     wi_len = 0.0f;
@@ -148,6 +153,8 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         pdf_pos_a = r.triangle_pdf;
         pdf_pos_w = pdf_pos_a * d2 / cos_from_light;
         rec.instance_idx = L.prim_mesh_idx;
+        n = r.n_s;
+        pos = r.pos;
     } else if (lt == LT_LIGHT_SPOT) {
         wi = ld3(L.pos) - p;
         const float d2 = dot(wi, wi);
@@ -159,6 +166,8 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         pdf_pos_a = 1.0f;
         pdf_pos_w = d2;
         Le = ld3(L.L) * fo;
+        n = -wi;
+        pos = ld3(L.pos);
     } else if (lt == LT_LIGHT_DIRECTIONAL) {
         const f3 dir = normalize(ld3(L.pos) - ld3(L.to));
         const f3 lp = p + dir * (2.0f * L.world_radius);
@@ -169,6 +178,8 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         pdf_pos_w = 1.0f;
         Le = ld3(L.L);
         cos_from_light = 1.0f;
+        n = -wi;
+        pos = lp;
     } else if (lt == LT_LIGHT_POINT) {
         wi = ld3(L.pos) - p;
         const float d2 = dot(wi, wi);
@@ -178,7 +189,114 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         pdf_pos_a = 1.0f;
         pdf_pos_w = d2;
         Le = ld3(L.L);
+        n = -wi;
+        pos = ld3(L.pos);
     }
+    return Le;
+}
+
+LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,
+                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec) {
+    f3 n, pos;
+    return sample_light_Li(sv, rp, p, pdf_pos_w, wi, wi_len, pdf_pos_a, cos_from_light, rec, n, pos);
+}
+
+// ---- emission sampling and light pdfs (BDPT / VCM) ----
+
+LT_DEV bool light_finite(uint32_t f) { return ((f >> 4) & 1u) != 0; }  // commons.glsl:42
+LT_DEV bool light_delta(uint32_t f) { return ((f >> 5) & 1u) != 0; }   // commons.glsl:44
+LT_DEV float uniform_cone_pdf(float cos_max) { return 1.0f / (TWO_PI * (1.0f - cos_max)); }  // commons.glsl:40
+
+// commons.glsl:81-95, the spot cone of light `li` (+ LT_LIGHT_POINT)
+LT_DEV float light_pdf_dir(const SceneView& sv, uint32_t li, uint32_t flags, f3 n_s, f3 wi) {
+    switch (flags & 7u) {
+        case LT_LIGHT_AREA: return fmaxf(dot(n_s, wi) / PI, 0.0f);
+        case LT_LIGHT_SPOT: return uniform_cone_pdf(sv.cones[2 * li]);
+        case LT_LIGHT_POINT: return 1.0f / (4.0f * PI);
+        default: return 0.0f;
+    }
+}
+
+// commons.glsl:64-79 (+ LT_LIGHT_POINT)
+LT_DEV float light_pdf_a_to_w(uint32_t flags, float pdf_a, float wi_len_sqr, float cos_from_light) {
+    switch (flags & 7u) {
+        case LT_LIGHT_AREA: return pdf_a * wi_len_sqr / cos_from_light;
+        case LT_LIGHT_SPOT: return wi_len_sqr / cos_from_light;
+        case LT_LIGHT_DIRECTIONAL: return 1.0f;
+        case LT_LIGHT_POINT: return wi_len_sqr;
+        default: return 0.0f;
+    }
+}
+
+// utils.glsl:157-173: rotation taking v to +z, and its application
+LT_DEV f4 to_local_quat(f3 v) {
+    if (v.z < -0.99999f) return f4{1.0f, 0.0f, 0.0f, 0.0f};
+    const float x = v.y, y = -v.x, z = 0.0f, w = 1.0f + v.z;
+    const float il = 1.0f / sqrtf(x * x + y * y + z * z + w * w);
+    return f4{x * il, y * il, z * il, w * il};
+}
+LT_DEV f3 rot_quat(f4 q, f3 v) {
+    const f3 qa = {q.x, q.y, q.z};
+    return 2.0f * dot(qa, v) * qa + (q.w * q.w - dot(qa, qa)) * v + 2.0f * q.w * cross(qa, v);
+}
+// commons.glsl:217-222
+LT_DEV f3 uniform_sample_cone(float u0, float u1, float cos_max) {
+    const float ct = (1.0f - u0) + u0 * cos_max;
+    const float st = sqrtf(1.0f - ct * ct);
+    const float phi = u1 * TWO_PI;
+    return {cosf(phi) * st, sinf(phi) * st, ct};
+}
+
+// commons.glsl:335-406 (+ LT_LIGHT_POINT: uniform sphere; spot cones per
+// light).  rp = rand4, rd = rand2.  pdf_pos_a is divided by total_light.
+LT_DEV f3 sample_light_Le(const SceneView& sv, f4 rp, float rd0, float rd1, float& cos_from_light, uint32_t& flags,
+                          uint32_t& li, f3& pos, f3& wi, f3& n, float& pdf_pos_a, float& pdf_dir_w) {
+    li = (uint32_t)(rp.x * (float)sv.num_lights);
+    const lt_light& L = sv.lights[li];
+    f3 Le = mk3(0.0f);
+    flags = L.light_flags;
+    cos_from_light = 0.0f;
+    pdf_pos_a = 0.0f;
+    pdf_dir_w = 0.0f;
+    pos = wi = n = mk3(0.0f);
+    const uint32_t lt = flags & 7u;
+    if (lt == LT_LIGHT_AREA) {
+        const lt_prim_mesh_info pi = sv.prims[L.prim_mesh_idx];
+        const uint32_t tri = (uint32_t)(rp.y * (float)L.num_triangles);
+        f2 bary;
+        const TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, tri, L.world_matrix, bary);
+        const Mat lm = load_material(sv, pi.material_index, bary);
+        pos = r.pos;
+        f3 T, B;
+        branchless_onb(r.n_s, T, B);
+        wi = to_world(sample_hemisphere(rd0, rd1), T, B, r.n_s);
+        Le = lm.emissive;
+        cos_from_light = fmaxf(dot(r.n_s, wi), 0.0f);
+        pdf_pos_a = r.triangle_pdf;
+        pdf_dir_w = dot(wi, r.n_s) / PI;
+        n = r.n_s;
+    } else if (lt == LT_LIGHT_SPOT) {
+        const float cw = sv.cones[2 * li], cf = sv.cones[2 * li + 1];
+        const f3 ld = normalize(ld3(L.to) - ld3(L.pos));
+        const f4 q = to_local_quat(ld);
+        wi = rot_quat(f4{-q.x, -q.y, -q.z, q.w}, uniform_sample_cone(rd0, rd1, cw));
+        pos = ld3(L.pos);
+        cos_from_light = dot(wi, ld);
+        Le = ld3(L.L) * spot_falloff(cos_from_light, cw, cf);
+        pdf_pos_a = 1.0f;
+        pdf_dir_w = uniform_cone_pdf(cw);
+        n = wi;
+    } else if (lt == LT_LIGHT_POINT) {
+        const float z = 1.0f - 2.0f * rd0, r = sqrtf(fmaxf(0.0f, 1.0f - z * z)), phi = TWO_PI * rd1;
+        wi = {r * cosf(phi), r * sinf(phi), z};
+        pos = ld3(L.pos);
+        cos_from_light = 1.0f;
+        Le = ld3(L.L);
+        pdf_pos_a = 1.0f;
+        pdf_dir_w = 1.0f / (4.0f * PI);
+        n = wi;
+    }
+    pdf_pos_a /= (float)sv.total_light_triangle_cnt;
     return Le;
 }
 

@@ -7,6 +7,7 @@
 #include <new>
 #include <string>
 
+#include "bdpt.h"
 #include "context.h"
 #include "path.h"
 #include "path_wave.h"
@@ -17,9 +18,12 @@ lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, siz
 struct lt_integrator {
     lt_context* ctx = nullptr;
     lt_integrator_kind kind = LT_INTEGRATOR_PATH;
+    uint32_t width = 0, height = 0;
     lt_path_config path{};
+    lt_bdpt_config bdpt{};
     float4* accum = nullptr;  // output_tex, RGBA32F, row-major
     WaveBuffers wave;         // wavefront state (LT_PATH_WAVEFRONT)
+    BdptBuffers bb;           // BDPT subpaths, camera estimates and splats
     int counting = 0;
     lt_render_stats last{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -32,23 +36,41 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
                                lt_integrator** out) {
     if (!ctx || !cfg || !out) return LT_ERR_INVALID_ARG;
     *out = nullptr;
-    if (kind != LT_INTEGRATOR_PATH)
-        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "only the Path integrator is built in this version");
-    if (cfg_size != sizeof(lt_path_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
-    const lt_path_config* pc = (const lt_path_config*)cfg;
-    if (pc->width == 0 || pc->height == 0 || pc->width > 16384 || pc->height > 16384)
-        return lt_fail(ctx, LT_ERR_INVALID_ARG, "image size out of range");
-    if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT)
-        return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown path strategy");
-    if (pc->strategy == LT_PATH_WAVEFRONT && pc->max_depth > kWaveMaxDepth)
-        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "wavefront path length is limited to 32");
+    uint32_t W = 0, H = 0;
+    if (kind == LT_INTEGRATOR_PATH) {
+        if (cfg_size != sizeof(lt_path_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
+        const lt_path_config* pc = (const lt_path_config*)cfg;
+        if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT)
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown path strategy");
+        if (pc->strategy == LT_PATH_WAVEFRONT && pc->max_depth > kWaveMaxDepth)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "wavefront path length is limited to 32");
+        W = pc->width;
+        H = pc->height;
+    } else if (kind == LT_INTEGRATOR_BDPT) {
+        if (cfg_size != sizeof(lt_bdpt_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
+        const lt_bdpt_config* bc = (const lt_bdpt_config*)cfg;
+        if (bc->max_depth < 1 || bc->max_depth > kBdptMaxDepth)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "BDPT path length must be in 1..16");
+        if (bc->mis != LT_MIS_BALANCE && bc->mis != LT_MIS_POWER)
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown MIS heuristic");
+        W = bc->width;
+        H = bc->height;
+    } else {
+        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "the VCM integrator is not built in this version");
+    }
+    if (W == 0 || H == 0 || W > 16384 || H > 16384) return lt_fail(ctx, LT_ERR_INVALID_ARG, "image size out of range");
     lt_integrator* it = new (std::nothrow) lt_integrator;
     if (!it) return LT_ERR_OUT_OF_MEMORY;
     it->ctx = ctx;
     it->kind = kind;
-    it->path = *pc;
+    it->width = W;
+    it->height = H;
+    if (kind == LT_INTEGRATOR_PATH)
+        it->path = *(const lt_path_config*)cfg;
+    else
+        it->bdpt = *(const lt_bdpt_config*)cfg;
     (void)hipSetDevice(ctx->device);
-    size_t bytes = (size_t)pc->width * pc->height * sizeof(float4);
+    size_t bytes = (size_t)W * H * sizeof(float4);
     hipError_t e = hipMalloc(&it->accum, bytes);
     if (e == hipSuccess) e = hipMemsetAsync(it->accum, 0, bytes, ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&it->e0);
@@ -57,6 +79,13 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
         lt_status st = lt_hip_check(ctx, e, "lt_integrator_create");
         lt_integrator_destroy(it);
         return st;
+    }
+    if (kind == LT_INTEGRATOR_BDPT) {
+        lt_status st = bdpt_alloc(ctx, it->bb, W, H, it->bdpt.max_depth);
+        if (st != LT_OK) {
+            lt_integrator_destroy(it);
+            return st;
+        }
     }
     *out = it;
     return LT_OK;
@@ -67,6 +96,7 @@ lt_status lt_integrator_destroy(lt_integrator* it) {
     if (it->ctx) (void)hipStreamSynchronize(it->ctx->stream);
     if (it->accum) (void)hipFree(it->accum);
     wave_free(it->wave);
+    bdpt_free(it->bb);
     if (it->e0) (void)hipEventDestroy(it->e0);
     if (it->e1) (void)hipEventDestroy(it->e1);
     delete it;
@@ -80,24 +110,39 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
     if (ctx->scene.nlights == 0 || ctx->scene.total_light_triangle_cnt == 0)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene has no lights");
     (void)hipSetDevice(ctx->device);
-    PathLaunch L{};
-    memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
-    memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
-    L.width = it->path.width;
-    L.height = it->path.height;
-    L.max_depth = it->path.max_depth;
-    memcpy(L.sky_col, it->path.sky_col, 12);
-    L.direct_lighting = it->path.direct_lighting;
-    L.frame_start = p->frame_num;
-    L.spp = p->spp ? p->spp : 1u;
-    L.seed = p->seed;
+    const uint32_t spp = p->spp ? p->spp : 1u;
     LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
-    lt_status st = it->path.strategy == LT_PATH_WAVEFRONT
-                       ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
-                       : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+    lt_status st;
+    if (it->kind == LT_INTEGRATOR_BDPT) {
+        BdptLaunch L{};
+        memcpy(L.view, p->camera.view, sizeof(L.view));
+        memcpy(L.projection, p->camera.projection, sizeof(L.projection));
+        memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+        memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+        L.width = it->width;
+        L.height = it->height;
+        L.max_depth = it->bdpt.max_depth;
+        L.power = it->bdpt.mis == LT_MIS_POWER ? 1u : 0u;
+        st = launch_bdpt(ctx, it->bb, L, p->frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
+    } else {
+        PathLaunch L{};
+        memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+        memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+        L.width = it->path.width;
+        L.height = it->path.height;
+        L.max_depth = it->path.max_depth;
+        memcpy(L.sky_col, it->path.sky_col, 12);
+        L.direct_lighting = it->path.direct_lighting;
+        L.frame_start = p->frame_num;
+        L.spp = spp;
+        L.seed = p->seed;
+        st = it->path.strategy == LT_PATH_WAVEFRONT
+                 ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
+                 : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+    }
     if (st != LT_OK) return st;
     LT_HIP(ctx, hipEventRecord(it->e1, ctx->stream));
-    it->last.frames = L.spp;
+    it->last.frames = spp;
     it->rendered = true;
     return LT_OK;
 }
@@ -133,14 +178,14 @@ lt_status lt_render_set_counting(lt_integrator* it, int enable) {
 lt_status lt_accum_reset(lt_integrator* it) {
     if (!it) return LT_ERR_INVALID_ARG;
     lt_context* ctx = it->ctx;
-    LT_HIP(ctx, hipMemsetAsync(it->accum, 0, (size_t)it->path.width * it->path.height * sizeof(float4), ctx->stream));
+    LT_HIP(ctx, hipMemsetAsync(it->accum, 0, (size_t)it->width * it->height * sizeof(float4), ctx->stream));
     return LT_OK;
 }
 
 lt_status lt_readback_rgba32f(lt_integrator* it, float* host, size_t n_floats) {
     if (!it || !host) return LT_ERR_INVALID_ARG;
     lt_context* ctx = it->ctx;
-    size_t need = (size_t)it->path.width * it->path.height * 4;
+    size_t need = (size_t)it->width * it->height * 4;
     if (n_floats < need) return lt_fail(ctx, LT_ERR_INVALID_ARG, "readback buffer too small");
     LT_HIP(ctx, hipMemcpyAsync(host, it->accum, need * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     LT_HIP(ctx, hipStreamSynchronize(ctx->stream));

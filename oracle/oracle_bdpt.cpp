@@ -39,7 +39,7 @@ static inline bool lt_delta(uint32_t f) { return ((f >> 5) & 1u) != 0; }
 static inline float uniform_cone_pdf(float cos_max) { return 1.0f / (TWO_PI * (1.0f - cos_max)); }
 
 static float spot_cos_width(const lt_scene_desc& s, uint32_t li) {
-    return s.spot_cones ? s.spot_cones[2 * li] : cosf(30.0f * PI / 180.0f);
+    return s.spot_cones ? s.spot_cones[2 * li] : cosf(PI / 6.0f);  // commons.glsl:49 cos(30 deg)
 }
 static float spot_cos_falloff(const lt_scene_desc& s, uint32_t li) {
     return s.spot_cones ? s.spot_cones[2 * li + 1] : cosf(25.0f * PI / 180.0f);
@@ -372,9 +372,12 @@ static bool connect_cam(BdptCtx& c, const PV* lv, const PV* cv, int s, vec3& out
     tg = {tg.x / tg.z, tg.y / tg.z, tg.z / tg.z, tg.w / tg.z};
     vec4 pr = mul(c.cam.projection, tg);
     pr = {-pr.x, -pr.y, -pr.z, -pr.w};
-    px = (int)(0.5f * (1.0f + pr.x) * (float)c.W - 0.5f);
-    py = (int)(0.5f * (1.0f + pr.y) * (float)c.H - 0.5f);
-    if (px < 0 || px >= (int)c.W || py < 0 || py >= (int)c.H || dot(dir, cv[0].n_s) < 0.0f) return false;
+    // ivec2() truncates toward zero: a coordinate in (-1, W) lands in [0, W)
+    const float fx = 0.5f * (1.0f + pr.x) * (float)c.W - 0.5f, fy = 0.5f * (1.0f + pr.y) * (float)c.H - 0.5f;
+    if (!(fx > -1.0f && fx < (float)c.W && fy > -1.0f && fy < (float)c.H) || dot(dir, cv[0].n_s) < 0.0f)
+        return false;
+    px = (int)fx;
+    py = (int)fy;
     float mis = 1.0f;
     if (luminance(L) != 0.0f) mis = calc_mis_weight(c, lv, cv, s, 1, sampled);
     out = mis * L;
