@@ -1,7 +1,10 @@
 """Headline benchmark: Path integrator throughput (Mrays/s) at 1920x1080 on
 one MI355X, on the procedural terrain workload (1.0M-triangle fBm heightfield
 + 2,000 instanced boxes, Disney materials, 64 point lights + 1 area light,
-Russian roulette after depth 3, max depth 12).
+Russian roulette after depth 3, max depth 12).  With --integrator bdpt it
+measures the BDPT caustics workload instead (20k-triangle glass sphere on a
+diffuse plane under a 15-degree spot light, 1024x1024, subpaths of up to 8
+bounces, 64 frames).
 
 A step is one 1-spp frame of the whole image (one lt_render launch).  Prints
 one JSON line with the metric, the roofline of the path kernel (algorithmic
@@ -42,6 +45,34 @@ def traffic_from_profile(workload: str):
     except Exception:
         return None
     return None
+
+
+def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float):
+    import oracle
+
+    log("cpu baseline: building host BVH")
+    acc = oracle.Accel(scene.desc)
+    threads = oracle.hardware_threads()
+    y0 = H // 2 - 16
+    _, secs, _, _ = acc.render_bdpt(camera, W, H, depth, frame_start=frame, frames=1, window=(0, y0, W, y0 + 32),
+                                    threads=threads)
+    per_row = max(secs / 32, 1e-5)
+    if per_row * H >= target_s:
+        rows, frames = int(max(8, target_s / per_row)), 1
+    else:
+        rows, frames = H, int(max(1, min(64, target_s / (per_row * H))))
+    y0 = max(0, H // 2 - rows // 2)
+    log(f"cpu baseline: {rows} rows x {W} px x {frames} frame(s), {threads} threads")
+    _, secs, cl, sh = acc.render_bdpt(camera, W, H, depth, frame_start=frame, frames=frames,
+                                      window=(0, y0, W, y0 + rows), threads=threads)
+    return {
+        "value": round((cl + sh) / secs / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"camera rows {y0}..{y0 + rows - 1} (with their light subpaths) of the {W}x{H} frames "
+                   f"{frame}..{frame + frames - 1} ({cl + sh} rays, {secs:.2f} s)"),
+    }
 
 
 def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
@@ -99,8 +130,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--integrator", choices=["path", "bdpt"], default="path")
+    ap.add_argument("--width", type=int, default=0, help="default 1920 (path) / 1024 (bdpt)")
+    ap.add_argument("--height", type=int, default=0, help="default 1080 (path) / 1024 (bdpt)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strategy", choices=["wavefront", "megakernel"], default="wavefront",
@@ -117,19 +149,28 @@ def main() -> int:
 
     import lumen_amd as la
 
-    W, H = args.width, args.height
-    log("generating terrain scene")
+    bdpt = args.integrator == "bdpt"
+    W = args.width or (1024 if bdpt else 1920)
+    H = args.height or (1024 if bdpt else 1080)
+    kind = "caustics" if bdpt else "terrain"
+    log(f"generating {kind} scene")
     t0 = time.time()
-    scene = la.Scene.procedural("terrain", seed=1)
+    scene = la.Scene.procedural(kind, seed=7 if bdpt else 1)
     log(f"scene: {scene.num_triangles} triangles in {time.time() - t0:.1f}s")
     ctx = la.Context(0)
     n_cu, arch = ctx.device_info()
-    it = la.Path(scene, ctx, W, H, strategy=args.strategy)
+    if bdpt:
+        it = la.BDPT(scene, ctx, W, H, path_length=8)
+        it.seed = 7
+    else:
+        it = la.Path(scene, ctx, W, H, strategy=args.strategy)
     it.init()
     bst = it.create_accel()
     log(f"bvh8: {bst.num_nodes} nodes, depth {bst.max_depth}, build {bst.build_ms:.1f} ms on {arch} ({n_cu} CUs)")
     depth = it.path_length
     sky = tuple(scene.integrator_cfg().sky_col)
+    if bdpt and args.steps == 32 and "--steps" not in sys.argv:
+        args.steps = 64  # the caustics workload is 64 spp
 
     for _ in range(args.warmup):
         it.render(spp=1)
@@ -152,12 +193,13 @@ def main() -> int:
     # untimed counting pass over the same frame indices (ray and BVH work counts
     # are a function of the frame index only)
     it.set_counting(True)
-    rays = nodes = tris = 0
+    rays = nodes = tris = closest = 0
     for k in range(args.steps):
         it.frame_num = first + k
         it.render(spp=1)
         s = it.stats()
         rays += s.closest_rays + s.shadow_rays
+        closest += s.closest_rays
         nodes += s.nodes_visited
         tris += s.tris_tested
     it.set_counting(False)
@@ -166,14 +208,17 @@ def main() -> int:
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     bytes_per_launch = (nodes * 128 + tris * 48) / args.steps
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    workload = "terrain-1M+2000-boxes-1920x1080-path-d12"
+    workload = (f"caustics-sphere20k-{W}x{H}-bdpt-d8" if bdpt else "terrain-1M+2000-boxes-1920x1080-path-d12")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
+        if bdpt:
+            cpu = cpu_baseline_bdpt(scene, it.camera, W, H, depth, first, args.cpu_seconds)
+        else:
+            cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
 
     out = {
-        "metric": "Mrays/s (Path, 1080p, 1 GPU)",
+        "metric": "Mrays/s (BDPT, 1 GPU)" if bdpt else "Mrays/s (Path, 1080p, 1 GPU)",
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -187,13 +232,15 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {
             "workload": workload,
-            "integrator": f"path-{args.strategy}",
+            "integrator": "bdpt" if bdpt else f"path-{args.strategy}",
             "width": W,
             "height": H,
             "spp_per_step": 1,
             "max_depth": depth,
             "triangles": scene.num_triangles,
             "rays_per_step": rays // args.steps,
+            "closest_rays_per_step": closest // args.steps,
+            "shadow_rays_per_step": (rays - closest) // args.steps,
         },
         "roofline": {
             "bound": "hbm",
@@ -202,7 +249,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(workload),
-            "kernel": "k_path_mega" if args.strategy == "megakernel" else "one frame's launch sequence (extend/shade/shadow/mis per depth)",
+            "kernel": ("one frame's launch sequence (walk + connect per pixel chunk, resolve)" if bdpt else
+                       "k_path_mega" if args.strategy == "megakernel" else
+                       "one frame's launch sequence (extend/shade/shadow/mis per depth)"),
             "kernel_ms_avg": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
