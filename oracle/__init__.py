@@ -36,6 +36,13 @@ class BdptParams(C.Structure):
                 ("x0", C.c_uint32), ("y0", C.c_uint32), ("x1", C.c_uint32), ("y1", C.c_uint32)]
 
 
+class VcmParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
+                ("frame_start", C.c_uint32), ("frames", C.c_uint32), ("seed", C.c_uint32),
+                ("radius_factor", C.c_float), ("alpha", C.c_float), ("use_vc", C.c_int32), ("use_vm", C.c_int32),
+                ("corrected", C.c_uint32), ("sky_col", C.c_float * 3)]
+
+
 _lib = None
 
 
@@ -78,6 +85,13 @@ def lib() -> C.CDLL:
                                       C.POINTER(C.c_uint64)]
         L.ora_render_bdpt.restype = C.c_double
         L.ora_sample_light_Le.argtypes = [P, P, P]
+        L.ora_render_vcm.argtypes = [P, P, P, C.POINTER(VcmParams), P, P, C.c_int, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint64)]
+        L.ora_render_vcm.restype = C.c_double
+        L.ora_vcm_grid_selfcheck.argtypes = [P, P, P, C.POINTER(VcmParams), C.c_int]
+        L.ora_vcm_words.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.ora_vcm_radius_grid.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int32)]
         L.ora_hardware_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -159,6 +173,19 @@ def sample_light_Le(desc, rands6):
     return out
 
 
+def vcm_words(seed: int, frame_num: int):
+    lw, ew = C.c_uint32(), C.c_uint32()
+    lib().ora_vcm_words(seed, frame_num, C.byref(lw), C.byref(ew))
+    return lw.value, ew.value
+
+
+def vcm_radius_grid(desc, radius_factor, alpha, frame_num):
+    r = C.c_float()
+    g = (C.c_int32 * 3)()
+    lib().ora_vcm_radius_grid(C.byref(desc), radius_factor, alpha, frame_num, C.byref(r), g)
+    return r.value, tuple(g)
+
+
 class Accel:
     """Host BVH2 over the scene's triangles (+ brute force)."""
 
@@ -223,6 +250,33 @@ class Accel:
                                      splat.ctypes.data, threads or lib().ora_hardware_threads(), C.byref(cl),
                                      C.byref(sh))
         return accum, secs, cl.value, sh.value
+
+    def render_vcm(self, camera, width, height, max_depth, frame_start=0, frames=1, seed=0, radius_factor=0.025,
+                   alpha=2.0 / 3.0, use_vc=True, use_vm=True, corrected=False, sky_col=(0, 0, 0), accum=None,
+                   splat=None, threads=0):
+        """Runs the reference VCM integrator (or its corrected form) on the CPU;
+        returns (image, seconds, closest, shadow)."""
+        p = VcmParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.frame_start, p.frames, p.seed = frame_start, frames, seed
+        p.radius_factor, p.alpha = radius_factor, alpha
+        p.use_vc, p.use_vm, p.corrected = int(use_vc), int(use_vm), int(corrected)
+        p.sky_col[:] = list(sky_col)
+        if accum is None:
+            accum = np.zeros((height, width, 4), np.float32)
+        if splat is None:
+            splat = np.zeros((height, width, 3), np.float32)
+        cl, sh = C.c_uint64(), C.c_uint64()
+        secs = lib().ora_render_vcm(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), accum.ctypes.data,
+                                    splat.ctypes.data, threads or lib().ora_hardware_threads(), C.byref(cl),
+                                    C.byref(sh))
+        return accum, secs, cl.value, sh.value
+
+    def vcm_grid_selfcheck(self, camera, width, height, max_depth, radius_factor, nq=200, seed=0):
+        p = VcmParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.seed, p.radius_factor, p.alpha = seed, radius_factor, 2.0 / 3.0
+        return lib().ora_vcm_grid_selfcheck(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), nq)
 
     def __del__(self):
         try:

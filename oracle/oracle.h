@@ -178,6 +178,11 @@ vec3 sample_light_Li(const lt_scene_desc& s, vec4 rands_pos, vec3 p, int num_lig
                      float& wi_len, float& pdf_pos_a, float& cos_from_light, LightRecord& rec, vec3& n, vec3& pos,
                      float& pdf_pos_dir_w);
 
+// commons.glsl:335-406 (+ point lights): one emission sample; pdf_pos_a is
+// divided by total_light (oracle_bdpt.cpp)
+vec3 sample_light_Le(const lt_scene_desc& s, vec4 rp, vec2 rd, int num_lights, int total_light, float& cos_from_light,
+                     uint32_t& flags, vec3& pos, vec3& wi, vec3& n, float& pdf_pos_a, float& pdf_dir_w);
+
 // ---- Path integrator (integrators/path/path.rgen, pt_commons.glsl) ----
 struct PathParams {
     uint32_t width, height;

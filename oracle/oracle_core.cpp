@@ -971,7 +971,7 @@ vec3 sample_light_Li(const lt_scene_desc& s, vec4 rp, vec3 p, int num_lights, fl
                                     : spot_falloff(cos_from_light);
             pdf_pos_a = 1.0f;
             pdf_pos_w = d2;
-            pdf_pos_dir_w = uniform_cone_pdf(cosf(PI / 6.0f));
+            pdf_pos_dir_w = uniform_cone_pdf(s.spot_cones ? s.spot_cones[2 * rec.light_idx] : cosf(PI / 6.0f));
             Le = load3(L.L) * fo;
             n = -wi;
             pos = load3(L.pos);
