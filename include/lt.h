@@ -175,7 +175,7 @@ typedef struct lt_ray {
     float tmax;
 } lt_ray;
 
-/* Closest-hit result (16 B).  prim = BLAS-local triangle index
+/* Closest-hit result (20 B).  prim = BLAS-local triangle index
  * (gl_PrimitiveID), inst = mesh index (gl_InstanceCustomIndexEXT); both are
  * 0xFFFFFFFF on a miss.  u, v are the barycentric weights of v1 and v2. */
 typedef struct lt_hit {
@@ -320,6 +320,29 @@ typedef struct lt_bdpt_config {
     uint32_t reserved[8];
 } lt_bdpt_config;
 
+/* VCM estimators. */
+#define LT_VCM_REFERENCE 0u /* the reference's shader: last-writer photon table, overwritten
+                               vertex-connection and merge results (vcm_commons.glsl:240,287,455) */
+#define LT_VCM_CORRECTED 1u /* summed connections and merges over every stored photon      */
+
+/* VCM integrator config: SceneConfig/VCMConfig (src/RayTracer/SceneConfig.h:38-42)
+ * plus the PCVCM fields set by VCM::render (src/RayTracer/VCM.cpp:131-158).
+ * One light subpath per pixel per frame; vertex connection is always on
+ * (VCM.cpp:7). */
+typedef struct lt_vcm_config {
+    uint32_t width;
+    uint32_t height;
+    int32_t max_depth;    /* path_length, 1..16 (default 6)                              */
+    float sky_col[3];
+    float radius_factor;  /* merge radius in percent of the scene's bounding radius
+                             (half diagonal), default 0.025                              */
+    float alpha;          /* radius reduction: r_n = r_0 / (n+1)^((1-alpha)/2); the
+                             reference hard-codes 2/3 (VCM.cpp:141)                      */
+    uint32_t enable_vm;   /* vertex merging (default 0)                                  */
+    uint32_t estimator;   /* LT_VCM_REFERENCE or LT_VCM_CORRECTED                         */
+    uint32_t reserved[6];
+} lt_vcm_config;
+
 /* Per-frame parameters (SceneUBO + push-constant frame_num). */
 typedef struct lt_frame_params {
     lt_camera camera;
@@ -342,7 +365,8 @@ typedef struct lt_render_stats {
 /* Replaces RayTracer::create_integrator + Integrator::init
  * (src/RayTracer/RayTracer.cpp:244-282, Integrator.cpp:8-50).  cfg points to
  * the config struct of `kind` (lt_path_config for LT_INTEGRATOR_PATH,
- * lt_bdpt_config for LT_INTEGRATOR_BDPT) and cfg_size must equal its size. */
+ * lt_bdpt_config for LT_INTEGRATOR_BDPT, lt_vcm_config for LT_INTEGRATOR_VCM)
+ * and cfg_size must equal its size. */
 lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const void* cfg, size_t cfg_size,
                                lt_integrator** out);
 lt_status lt_integrator_destroy(lt_integrator* it);

@@ -17,7 +17,7 @@ from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRA
                    LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS, LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL,
                    LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
 
-__all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
+__all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "VCM", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
            "lib", "make_rays"]
 
 _KINDS = {"cornell": LT_SCENE_CORNELL, "terrain": LT_SCENE_TERRAIN, "caustics": LT_SCENE_CAUSTICS,
@@ -360,6 +360,39 @@ class BDPT(Integrator):
         return c
 
 
+class VCM(Integrator):
+    """Vertex connection and merging (src/RayTracer/VCM.{h,cpp}): one light
+    subpath per pixel per frame, camera splats, NEE, connections to one
+    random light subpath, and (enable_vm) photon merging over a hash grid
+    whose radius shrinks as r_0 / (n+1)^((1-alpha)/2).  estimator="reference"
+    reproduces the reference shader; "corrected" sums every connection and
+    merge."""
+
+    kind = LT_INTEGRATOR_VCM
+
+    def __init__(self, lumen_scene, ctx, width=1920, height=1080, path_length=None, sky_col=None,
+                 radius_factor=None, alpha=2.0 / 3.0, enable_vm=None, estimator="reference"):
+        super().__init__(lumen_scene, ctx, width, height)
+        icfg = lumen_scene.integrator_cfg()
+        self.path_length = path_length if path_length is not None else (icfg.path_length or 6)
+        self.sky_col = sky_col if sky_col is not None else tuple(icfg.sky_col)
+        self.radius_factor = radius_factor if radius_factor is not None else (icfg.radius_factor or 0.025)
+        self.alpha = alpha
+        self.enable_vm = bool(icfg.enable_vm) if enable_vm is None else bool(enable_vm)
+        self.estimator = {"reference": _abi.LT_VCM_REFERENCE, "corrected": _abi.LT_VCM_CORRECTED}[estimator]
+
+    def _config(self):
+        c = _abi.lt_vcm_config()
+        c.width, c.height = self.width, self.height
+        c.max_depth = self.path_length
+        c.sky_col[:] = list(self.sky_col)
+        c.radius_factor = self.radius_factor
+        c.alpha = self.alpha
+        c.enable_vm = int(self.enable_vm)
+        c.estimator = self.estimator
+        return c
+
+
 def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 1920, height: int = 1080,
                       **kw) -> Integrator:
     """RayTracer::create_integrator (RayTracer.cpp:244-282), keyed by the
@@ -369,6 +402,8 @@ def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 
         return Path(lumen_scene, ctx, width, height, **kw)
     if n == "bdpt":
         return BDPT(lumen_scene, ctx, width, height, **kw)
+    if n == "vcm":
+        return VCM(lumen_scene, ctx, width, height, **kw)
     raise LtError(6, f"integrator '{name}' is not built")
 
 

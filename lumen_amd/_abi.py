@@ -29,6 +29,7 @@ LT_PTR_DEVICE = 1
 LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM = 0, 1, 2
 LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT = 0, 1
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
+LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
 
@@ -96,6 +97,12 @@ class lt_path_config(C.Structure):
 class lt_bdpt_config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("mis", C.c_uint32),
                 ("reserved", C.c_uint32 * 8)]
+
+
+class lt_vcm_config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("sky_col", C.c_float * 3),
+                ("radius_factor", C.c_float), ("alpha", C.c_float), ("enable_vm", C.c_uint32),
+                ("estimator", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 class lt_frame_params(C.Structure):

@@ -124,9 +124,13 @@ LT_DEV float spot_falloff(float cos_from_light, float cos_width, float cos_fallo
 }
 
 // commons.glsl:224-300 (+ LT_LIGHT_POINT).  rp = rand4; n and pos are the
-// sampled light point's normal and position (commons.glsl:302-307).
+// sampled light point's normal and position (commons.glsl:302-307) and
+// pdf_pos_dir_w the pdf of emitting along -wi from it (the light's cone for
+// spot lights).
 LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,
-                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec, f3& n, f3& pos) {
+                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec, f3& n, f3& pos,
+                          float& pdf_pos_dir_w) {
+    pdf_pos_dir_w = 0.0f;
     rec.light_idx = (uint32_t)(rp.x * (float)sv.num_lights);
     const lt_light& L = sv.lights[rec.light_idx];
     const uint32_t flags = L.light_flags;
@@ -152,6 +156,7 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         Le = lm.emissive;
         pdf_pos_a = r.triangle_pdf;
         pdf_pos_w = pdf_pos_a * d2 / cos_from_light;
+        pdf_pos_dir_w = cos_from_light * INV_PI * r.triangle_pdf;
         rec.instance_idx = L.prim_mesh_idx;
         n = r.n_s;
         pos = r.pos;
@@ -165,6 +170,7 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         const float fo = spot_falloff(cos_from_light, sv.cones[2 * rec.light_idx], sv.cones[2 * rec.light_idx + 1]);
         pdf_pos_a = 1.0f;
         pdf_pos_w = d2;
+        pdf_pos_dir_w = 1.0f / (TWO_PI * (1.0f - sv.cones[2 * rec.light_idx]));
         Le = ld3(L.L) * fo;
         n = -wi;
         pos = ld3(L.pos);
@@ -176,6 +182,7 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         wi /= wi_len;
         pdf_pos_a = 1.0f;
         pdf_pos_w = 1.0f;
+        pdf_pos_dir_w = INV_PI / (L.world_radius * L.world_radius);
         Le = ld3(L.L);
         cos_from_light = 1.0f;
         n = -wi;
@@ -188,11 +195,18 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
         cos_from_light = 1.0f;
         pdf_pos_a = 1.0f;
         pdf_pos_w = d2;
+        pdf_pos_dir_w = 1.0f / (4.0f * PI);
         Le = ld3(L.L);
         n = -wi;
         pos = ld3(L.pos);
     }
     return Le;
+}
+
+LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,
+                          float& pdf_pos_a, float& cos_from_light, LightRecord& rec, f3& n, f3& pos) {
+    float pdw;
+    return sample_light_Li(sv, rp, p, pdf_pos_w, wi, wi_len, pdf_pos_a, cos_from_light, rec, n, pos, pdw);
 }
 
 LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3& wi, float& wi_len,

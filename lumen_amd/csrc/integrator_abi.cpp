@@ -12,6 +12,7 @@
 #include "path.h"
 #include "path_wave.h"
 #include "scene.h"
+#include "vcm.h"
 
 lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, size_t n, double* d_out);
 
@@ -21,9 +22,11 @@ struct lt_integrator {
     uint32_t width = 0, height = 0;
     lt_path_config path{};
     lt_bdpt_config bdpt{};
+    lt_vcm_config vcm{};
     float4* accum = nullptr;  // output_tex, RGBA32F, row-major
     WaveBuffers wave;         // wavefront state (LT_PATH_WAVEFRONT)
     BdptBuffers bb;           // BDPT subpaths, camera estimates and splats
+    VcmBuffers vb;            // VCM light vertices, photon grid and splats
     int counting = 0;
     lt_render_stats last{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -55,8 +58,22 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
             return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown MIS heuristic");
         W = bc->width;
         H = bc->height;
+    } else if (kind == LT_INTEGRATOR_VCM) {
+        if (cfg_size != sizeof(lt_vcm_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
+        const lt_vcm_config* vc = (const lt_vcm_config*)cfg;
+        if (vc->max_depth < 1 || vc->max_depth > kVcmMaxDepth)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "VCM path length must be in 1..16");
+        if (vc->estimator != LT_VCM_REFERENCE && vc->estimator != LT_VCM_CORRECTED)
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown VCM estimator");
+        if (!(vc->radius_factor > 0.0f) || !(vc->alpha > 0.0f && vc->alpha <= 1.0f))
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "VCM radius_factor must be > 0 and alpha in (0, 1]");
+        if ((uint64_t)vc->width * vc->height * 10u >= (1ull << 31) ||
+            (uint64_t)vc->width * vc->height * (vc->max_depth + 1u) >= (1ull << 32) - 1u)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "VCM image too large for 32-bit photon references");
+        W = vc->width;
+        H = vc->height;
     } else {
-        return lt_fail(ctx, LT_ERR_UNSUPPORTED, "the VCM integrator is not built in this version");
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown integrator kind");
     }
     if (W == 0 || H == 0 || W > 16384 || H > 16384) return lt_fail(ctx, LT_ERR_INVALID_ARG, "image size out of range");
     lt_integrator* it = new (std::nothrow) lt_integrator;
@@ -67,8 +84,10 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
     it->height = H;
     if (kind == LT_INTEGRATOR_PATH)
         it->path = *(const lt_path_config*)cfg;
-    else
+    else if (kind == LT_INTEGRATOR_BDPT)
         it->bdpt = *(const lt_bdpt_config*)cfg;
+    else
+        it->vcm = *(const lt_vcm_config*)cfg;
     (void)hipSetDevice(ctx->device);
     size_t bytes = (size_t)W * H * sizeof(float4);
     hipError_t e = hipMalloc(&it->accum, bytes);
@@ -80,8 +99,9 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
         lt_integrator_destroy(it);
         return st;
     }
-    if (kind == LT_INTEGRATOR_BDPT) {
-        lt_status st = bdpt_alloc(ctx, it->bb, W, H, it->bdpt.max_depth);
+    if (kind == LT_INTEGRATOR_BDPT || kind == LT_INTEGRATOR_VCM) {
+        lt_status st = kind == LT_INTEGRATOR_BDPT ? bdpt_alloc(ctx, it->bb, W, H, it->bdpt.max_depth)
+                                                  : vcm_alloc(ctx, it->vb, W, H, it->vcm.max_depth);
         if (st != LT_OK) {
             lt_integrator_destroy(it);
             return st;
@@ -97,6 +117,7 @@ lt_status lt_integrator_destroy(lt_integrator* it) {
     if (it->accum) (void)hipFree(it->accum);
     wave_free(it->wave);
     bdpt_free(it->bb);
+    vcm_free(it->vb);
     if (it->e0) (void)hipEventDestroy(it->e0);
     if (it->e1) (void)hipEventDestroy(it->e1);
     delete it;
@@ -124,6 +145,22 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
         L.max_depth = it->bdpt.max_depth;
         L.power = it->bdpt.mis == LT_MIS_POWER ? 1u : 0u;
         st = launch_bdpt(ctx, it->bb, L, p->frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
+    } else if (it->kind == LT_INTEGRATOR_VCM) {
+        VcmLaunch L{};
+        memcpy(L.view, p->camera.view, sizeof(L.view));
+        memcpy(L.projection, p->camera.projection, sizeof(L.projection));
+        memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+        memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+        L.width = it->width;
+        L.height = it->height;
+        L.n_paths = it->width * it->height;
+        L.max_depth = it->vcm.max_depth;
+        memcpy(L.bmin, ctx->scene.smin, 12);
+        memcpy(L.bmax, ctx->scene.smax, 12);
+        memcpy(L.sky_col, it->vcm.sky_col, 12);
+        L.use_vm = it->vcm.enable_vm ? 1u : 0u;
+        L.corrected = it->vcm.estimator == LT_VCM_CORRECTED ? 1u : 0u;
+        st = launch_vcm(ctx, it->vb, L, it->vcm, p->frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
     } else {
         PathLaunch L{};
         memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));

@@ -1,0 +1,725 @@
+// vcm.hip — the VCM integrator on gfx950.
+//
+// A frame is a light pass, the photon grid, and an eye pass:
+//   k_vcm_light   one light subpath per pixel slot (vcm_light.rgen:123-166,
+//                 vcm_commons.glsl:77-130, 302-474): stores its non-specular
+//                 vertices as 96-B records, splats camera connections with
+//                 float atomics and counts its photons into hash buckets;
+//   grid          reference estimator: each bucket keeps the largest vertex
+//                 reference inserted (atomicMax, a deterministic "last
+//                 writer") and the insertion count; corrected estimator: an
+//                 exclusive scan of the bucket counts and a scatter of every
+//                 photon reference (k_vcm_scatter);
+//   k_vcm_eye     one camera path per pixel (vcm_eye.rgen:42-98,
+//                 vcm_commons.glsl:477-631): emission, NEE, connections to the
+//                 vertices of one random light path, merging over the hash
+//                 grid; then the pixel's splats and the running mean.
+// All three are persistent: waves pull 64-item chunks from a counter.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "dev_scene.h"
+#include "dev_trace.h"
+#include "vcm.h"
+
+using namespace ltd;
+
+namespace {
+
+constexpr float VCM_TMIN = 0.001f;    // vcm_light.rgen:54
+constexpr float VCM_TMAX = 10000.0f;  // vcm_light.rgen:55
+
+struct Ctr {
+    uint32_t closest, shadow, nodes, tris;
+};
+
+struct VState {  // VCMState (utils.glsl:28-39)
+    f3 wi, n_s, pos, thr;
+    float d_vcm, d_vc, d_vm;
+};
+
+struct LVtx {  // a stored light vertex in registers
+    f3 pos, n_s, wi, wo, thr;
+    f2 uv;
+    float d_vcm, d_vc, d_vm;
+    uint32_t mat, path_len, side;
+};
+
+LT_DEV LVtx load_lv(const VcmVertex* __restrict__ v) {
+    const float4 a = v->a, b = v->b, c = v->c, d = v->d, e = v->e;
+    const uint4 f = v->f;
+    LVtx r;
+    r.pos = {a.x, a.y, a.z};
+    r.d_vcm = a.w;
+    r.n_s = {b.x, b.y, b.z};
+    r.d_vc = b.w;
+    r.wi = {c.x, c.y, c.z};
+    r.d_vm = c.w;
+    r.wo = {d.x, d.y, d.z};
+    r.thr = {e.x, e.y, e.z};
+    r.uv = {d.w, e.w};
+    r.mat = f.x;
+    r.path_len = f.y;
+    r.side = f.z;
+    return r;
+}
+
+LT_DEV bool is_zero(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+
+// utils.glsl:62-65 (GLSL int products wrap like uint32)
+LT_DEV uint32_t cell_hash(int x, int y, int z, uint32_t n) {
+    const uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+    return h % (10u * n);
+}
+// vcm_commons.glsl:19-23 (its clamp result is discarded, so no clamp here)
+LT_DEV void grid_idx(const VcmLaunch& L, f3 p, int o[3]) {
+    const float qx = (p.x - L.bmin[0]) / (L.bmax[0] - L.bmin[0]);
+    const float qy = (p.y - L.bmin[1]) / (L.bmax[1] - L.bmin[1]);
+    const float qz = (p.z - L.bmin[2]) / (L.bmax[2] - L.bmin[2]);
+    o[0] = (int)(qx * (float)L.grid_res[0]);
+    o[1] = (int)(qy * (float)L.grid_res[1]);
+    o[2] = (int)(qz * (float)L.grid_res[2]);
+}
+
+template <bool COUNT>
+LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk, Ctr& c) {
+    if (COUNT) c.shadow++;
+    HitRec h;
+    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.nodes, c.tris);
+}
+
+template <bool COUNT>
+LT_DEV bool closest(const SceneView& sv, f3 o, f3 d, Payload& p, const TraceStack& stk, Ctr& c) {
+    if (COUNT) c.closest++;
+    HitRec h;
+    if (!trace_bvh<false, COUNT>(sv, o, d, VCM_TMIN, VCM_TMAX, h, stk, c.nodes, c.tris)) return false;
+    fill_payload(sv, h.inst, h.prim, h.u, h.v, p);
+    return true;
+}
+
+struct Camera {
+    f3 pos, nrm;
+    float area;
+};
+LT_DEV Camera camera_of(const VcmLaunch& L) {
+    const f4 o4 = mul4(L.inv_view, f4{0.0f, 0.0f, 0.0f, 1.0f});
+    const f4 fw = mul4(L.inv_view, f4{0.0f, 0.0f, 1.0f, 0.0f});
+    f4 ai = mul4(L.inv_projection, f4{2.0f / (float)L.width, 2.0f / (float)L.height, 0.0f, 1.0f});
+    ai = f4{ai.x / ai.w, ai.y / ai.w, ai.z / ai.w, ai.w / ai.w};
+    return Camera{{o4.x, o4.y, o4.z}, {-fw.x, -fw.y, -fw.z}, fabsf(ai.x * ai.y)};
+}
+
+// ---------------------------------------------------------------- light pass
+// vcm_commons.glsl:24-75; returns the splat pixel or -1
+template <bool COUNT>
+LT_DEV int connect_cam(const SceneView& sv, const VcmLaunch& L, const Camera& cam, f3 n_s, f3 payload_n_s, f3 pos,
+                       const VState& st, f3 wo, const Mat& mat, f3& out, const TraceStack& stk, Ctr& c) {
+    const float screen_size = (float)L.n_paths;
+    f3 dir = cam.pos - pos;
+    const float len = length(dir);
+    dir /= len;
+    const float cos_y = dot(dir, n_s);
+    const float cos_theta = dot(cam.nrm, -dir);
+    if (cos_theta <= 0.0f) return -1;
+    const float c3 = cos_theta * cos_theta * cos_theta;
+    const float ratio = fabsf(cos_y) / (cam.area * c3 * len * len);
+    float pdf_rev, pdf_fwd;
+    const f3 f = eval_bsdf(n_s, wo, mat, dot(payload_n_s, wo) > 0.0f, dir, pdf_fwd, pdf_rev, true);
+    if (is_zero(f)) return -1;
+    // raster position first (no state beyond it is needed by the query)
+    const f3 nd = -dir;
+    f4 tg = mul4(L.view, f4{nd.x, nd.y, nd.z, 0.0f});
+    tg = f4{tg.x / tg.z, tg.y / tg.z, tg.z / tg.z, tg.w / tg.z};
+    const f4 pr = mul4(L.projection, tg);
+    float fx, fy;
+    {
+#pragma clang fp contract(off)
+        fx = 0.5f * (1.0f - pr.x) * (float)L.width - 0.5f;
+        fy = 0.5f * (1.0f - pr.y) * (float)L.height - 0.5f;
+    }
+    // ivec2() truncates toward zero: a coordinate in (-1, W) lands in [0, W)
+    if (!(fx > -1.0f && fx < (float)L.width && fy > -1.0f && fy < (float)L.height) || dot(nd, cam.nrm) < 0.0f)
+        return -1;
+    if (!(ratio > 0.0f)) return -1;
+    const float w_light = (ratio / screen_size) * (L.eta_vm + st.d_vcm + pdf_rev * st.d_vc);
+    const float mis = 1.0f / (1.0f + w_light);
+    out = mis * st.thr * ratio * f / screen_size;
+    if (!(luminance(out) > 0.0f)) return -1;
+    if (occluded<COUNT>(sv, offset_ray2(pos, n_s), dir, len - EPS, stk, c)) return -1;
+    return (int)fy * (int)L.width + (int)fx;
+}
+
+template <bool COUNT>
+LT_DEV uint32_t light_path(const SceneView& sv, const VcmLaunch& L, const Camera& cam, uint32_t p,
+                           VcmVertex* __restrict__ lv, float* __restrict__ splat, const TraceStack& stk, Ctr& c) {
+    const uint32_t x = p / L.height, y = p % L.height;  // pixel_idx = x * H + y (vcm_light.rgen:56)
+    RNG seed = {x, y, L.light_word, 0u};
+    f4 rp;
+    rp.x = rnd(seed);
+    rp.y = rnd(seed);
+    rp.z = rnd(seed);
+    rp.w = rnd(seed);
+    const float rd0 = rnd(seed), rd1 = rnd(seed);
+    float cos_theta, pdf_pos, pdf_dir;
+    uint32_t flags, li;
+    f3 pos, wi, n;
+    const f3 Le = sample_light_Le(sv, rp, rd0, rd1, cos_theta, flags, li, pos, wi, n, pdf_pos, pdf_dir);
+    if (pdf_dir <= 0.0f) return 0;
+    VState st;
+    const float pdf_emit = pdf_pos * pdf_dir, pdf_direct = pdf_pos;
+    st.pos = pos;
+    st.wi = wi;
+    st.n_s = n;
+    st.thr = Le * cos_theta / (pdf_dir * pdf_pos);
+    st.d_vcm = pdf_direct / pdf_emit;
+    const bool finite = light_finite(flags);
+    st.d_vc = light_delta(flags) ? 0.0f : (finite ? cos_theta : 1.0f) / (pdf_dir * pdf_pos);
+    st.d_vm = st.d_vc * L.eta_vc;
+    uint32_t path_idx = 0;
+    for (int depth = 1;; depth++) {
+        Payload pl;
+        if (!closest<COUNT>(sv, st.pos, st.wi, pl, stk, c)) break;
+        f3 wo = st.pos - pl.pos;
+        f3 n_s = pl.n_s, n_g = pl.n_g;
+        bool side = true;
+        if (dot(pl.n_g, wo) <= 0.0f) n_g = -n_g;
+        if (dot(n_g, n_s) < 0.0f) {
+            n_s = -n_s;
+            side = false;
+        }
+        const float dist = length(pl.pos - st.pos);
+        const float dist_sqr = dist * dist;
+        wo /= dist;
+        const Mat mat = load_material(sv, pl.material_idx, pl.uv);
+        const bool spec = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
+        const float cos_theta_wo = fabsf(dot(wo, n_s));
+        if (depth > 1 || finite) st.d_vcm *= dist_sqr;
+        st.d_vcm /= cos_theta_wo;
+        st.d_vc /= cos_theta_wo;
+        st.d_vm /= cos_theta_wo;
+        if (!spec) {  // vertex connection is always on (VCM.cpp:7)
+            VcmVertex* v = lv + path_idx;
+            v->a = make_float4(pl.pos.x, pl.pos.y, pl.pos.z, st.d_vcm);
+            v->b = make_float4(n_s.x, n_s.y, n_s.z, st.d_vc);
+            v->c = make_float4(st.wi.x, st.wi.y, st.wi.z, st.d_vm);
+            v->d = make_float4(wo.x, wo.y, wo.z, pl.uv.x);
+            v->e = make_float4(st.thr.x, st.thr.y, st.thr.z, pl.uv.y);
+            v->f = make_uint4(pl.material_idx, (uint32_t)depth + 1u, side ? 1u : 0u, 0u);
+            path_idx++;
+        }
+        if (depth >= L.max_depth) break;
+        if (!spec) {
+            f3 sc;
+            const int px = connect_cam<COUNT>(sv, L, cam, n_s, pl.n_s, pl.pos, st, wo, mat, sc, stk, c);
+            if (px >= 0) {
+                float* dst = splat + 3 * (size_t)px;
+                unsafeAtomicAdd(dst, sc.x);
+                unsafeAtomicAdd(dst + 1, sc.y);
+                unsafeAtomicAdd(dst + 2, sc.z);
+            }
+        }
+        float pdf_d, cos_t;
+        const float r0 = rnd(seed), r1 = rnd(seed), r2 = rnd(seed);
+        const f3 f = sample_bsdf(n_s, wo, mat, 0u, side, st.wi, pdf_d, cos_t, r0, r1, r2);
+        const bool same_hem = dot(st.wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
+        const bool transm = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
+        if (is_zero(f) || pdf_d == 0.0f || (!same_hem && !transm)) break;
+        float pdf_rev = pdf_d;
+        if (!spec) pdf_rev = bsdf_pdf(mat, n_s, st.wi, wo, side);
+        const float ac = fabsf(cos_t);
+        st.pos = offset_ray(pl.pos, n_g);
+        if (!spec) {
+            st.d_vc = (ac / pdf_d) * (L.eta_vm + st.d_vcm + pdf_rev * st.d_vc);
+            st.d_vm = (ac / pdf_d) * (1.0f + st.d_vcm * L.eta_vc + pdf_rev * st.d_vm);
+            st.d_vcm = 1.0f / pdf_d;
+        } else {
+            st.d_vcm = 0.0f;
+            st.d_vc *= ac;
+            st.d_vm *= ac;
+        }
+        st.thr *= f * ac / pdf_d;
+        st.n_s = n_s;
+    }
+    return path_idx;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_vcm_light(SceneView sv, VcmLaunch L, VcmVertex* __restrict__ verts,
+                                                           uint32_t* __restrict__ cnt, float* __restrict__ splat,
+                                                           uint32_t* __restrict__ key, uint32_t* __restrict__ count,
+                                                           uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+                                                           unsigned long long* __restrict__ counts) {
+    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t chunks = (L.n_paths + 63u) / 64u;
+    const uint32_t nv = (uint32_t)L.max_depth + 1u;
+    const Camera cam = camera_of(L);
+    Ctr c{0, 0, 0, 0};
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        t = __shfl(t, 0);
+        if (t >= chunks) break;
+        const uint32_t p = t * 64u + lane;
+        if (p >= L.n_paths) continue;
+        VcmVertex* lv = verts + (size_t)p * nv;
+        const uint32_t n = light_path<COUNT>(sv, L, cam, p, lv, splat, stk, c);
+        cnt[p] = n;
+        if (L.use_vm) {  // vcm_commons.glsl:455-469
+            for (uint32_t i = 0; i < n; i++) {
+                const float4 a = lv[i].a;
+                int g[3];
+                grid_idx(L, f3{a.x, a.y, a.z}, g);
+                const uint32_t h = cell_hash(g[0], g[1], g[2], L.n_paths);
+                if (!L.corrected) atomicMax(&key[h], p * nv + i + 1u);
+                atomicAdd(&count[h], 1u);
+            }
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, tt = c.tris;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+            n += __shfl_xor(n, o);
+            tt += __shfl_xor(tt, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&counts[0], a);
+            atomicAdd(&counts[1], b);
+            atomicAdd(&counts[2], n);
+            atomicAdd(&counts[3], tt);
+        }
+    }
+}
+
+// corrected estimator: every photon reference into its bucket's range
+__global__ void k_vcm_scatter(VcmLaunch L, const VcmVertex* __restrict__ verts, const uint32_t* __restrict__ cnt,
+                              const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
+                              uint32_t* __restrict__ items) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L.n_paths) return;
+    const uint32_t nv = (uint32_t)L.max_depth + 1u;
+    for (uint32_t i = 0; i < cnt[p]; i++) {
+        const float4 a = verts[(size_t)p * nv + i].a;
+        int g[3];
+        grid_idx(L, f3{a.x, a.y, a.z}, g);
+        const uint32_t h = cell_hash(g[0], g[1], g[2], L.n_paths);
+        items[start[h] + atomicAdd(&fill[h], 1u)] = p * nv + i;
+    }
+}
+
+// ------------------------------------------------------------------ eye pass
+struct Grid {
+    const VcmVertex* verts;
+    const uint32_t *cnt, *key, *count, *start, *items;
+};
+
+// vcm_commons.glsl:132-143
+LT_DEV f3 light_radiance(const SceneView& sv, const VcmLaunch& L, const Mat& mat, const VState& cs, const Payload& pl,
+                         int d) {
+    if (d == 1) return mat.emissive;
+    const float pdf_light_pos = 1.0f / (pl.area * (float)sv.total_light_triangle_cnt);
+    const float pdf_light_dir = fabsf(dot(pl.n_s, -cs.wi)) / PI;
+    const float w_camera = pdf_light_pos * cs.d_vcm + (pdf_light_pos * pdf_light_dir) * cs.d_vc;
+    const float mis = 1.0f / (1.0f + w_camera);
+    return mis * mat.emissive;
+}
+
+// vcm_commons.glsl:145-199
+template <bool COUNT>
+LT_DEV f3 connect_light(const SceneView& sv, const VcmLaunch& L, f3 n_s, f3 wo, const Mat& mat, bool side,
+                        const VState& cs, const Payload& pl, RNG& seed, float& pdf_rev, const TraceStack& stk,
+                        Ctr& c) {
+    f4 r4;
+    r4.x = rnd(seed);
+    r4.y = rnd(seed);
+    r4.z = rnd(seed);
+    r4.w = rnd(seed);
+    f3 wi, nn, lp;
+    float wi_len, pdf_pos_w, pdf_pos_a, cos_y, pdf_pos_dir_w;
+    LightRecord rec;
+    rec.triangle_idx = rec.instance_idx = 0u;
+    const f3 Le =
+        sample_light_Li(sv, r4, pl.pos, pdf_pos_w, wi, wi_len, pdf_pos_a, cos_y, rec, nn, lp, pdf_pos_dir_w);
+    const float cos_x = dot(wi, n_s);
+    float pdf_fwd;
+    const f3 f = eval_bsdf(n_s, wo, mat, side, wi, pdf_fwd, pdf_rev, true);
+    if (is_zero(f)) return mk3(0.0f);
+    if (light_delta(rec.flags)) pdf_fwd = 0.0f;
+    const float ltc = (float)sv.total_light_triangle_cnt;
+    const float w_light = pdf_fwd / (pdf_pos_w / ltc);
+    const float w_cam = pdf_pos_dir_w * fabsf(cos_x) / (pdf_pos_w * cos_y) * (L.eta_vm + cs.d_vcm + cs.d_vc * pdf_rev);
+    const float mis = 1.0f / (1.0f + w_light + w_cam);
+    if (!(mis > 0.0f)) return mk3(0.0f);
+    const f3 res = mis * fabsf(cos_x) * f * cs.thr * Le / (pdf_pos_w / ltc);
+    if (occluded<COUNT>(sv, offset_ray2(pl.pos, n_s), wi, wi_len - EPS, stk, c)) return mk3(0.0f);
+    return res;
+}
+
+// vcm_commons.glsl:201-247 over the stored vertices of light path lp
+template <bool COUNT>
+LT_DEV f3 connect_light_vertices(const SceneView& sv, const VcmLaunch& L, const Grid& G, uint32_t lp, int depth,
+                                 f3 n_s, f3 wo, const Mat& mat, bool side, const VState& cs, float nee_pdf_rev,
+                                 const Payload& pl, const TraceStack& stk, Ctr& c) {
+    f3 res = mk3(0.0f);
+    const uint32_t nv = (uint32_t)L.max_depth + 1u;
+    const uint32_t n = G.cnt[lp];
+    for (uint32_t i = 0; i < n; i++) {
+        const LVtx v = load_lv(G.verts + (size_t)lp * nv + i);
+        const uint32_t md = v.path_len + (uint32_t)depth - 1u;
+        if (md >= (uint32_t)L.max_depth) break;
+        f3 dir = v.pos - pl.pos;
+        const float len = length(dir);
+        const float len_sqr = len * len;
+        dir /= len;
+        const float cos_cam = dot(n_s, dir), cos_light = dot(v.n_s, -dir);
+        const float Gt = cos_light * cos_cam / len_sqr;
+        if (!(Gt > 0.0f)) continue;
+        float cam_pdf_fwd, cam_pdf_rev, l_pdf_fwd, l_pdf_rev;
+        const f3 f_cam = eval_bsdf(n_s, wo, mat, side, dir, cam_pdf_fwd, cam_pdf_rev, L.corrected != 0u);
+        f3 f_light;
+        {
+            const Mat lm = load_material(sv, v.mat, v.uv);
+            f_light = eval_bsdf(v.n_s, v.wo, lm, v.side == 1u, -dir, l_pdf_fwd, l_pdf_rev, true);
+        }
+        if (is_zero(f_light) || is_zero(f_cam)) continue;
+        cam_pdf_fwd *= fabsf(cos_light) / len_sqr;
+        l_pdf_fwd *= fabsf(cos_cam) / len_sqr;
+        const float w_light = cam_pdf_fwd * (L.eta_vm + v.d_vcm + l_pdf_rev * v.d_vc);
+        const float rev = L.corrected ? cam_pdf_rev : nee_pdf_rev;
+        const float w_camera = l_pdf_fwd * (L.eta_vm + cs.d_vcm + rev * cs.d_vc);
+        const float mis = 1.0f / (1.0f + w_camera + w_light);
+        const f3 r = mis * Gt * cs.thr * v.thr * f_cam * f_light;
+        if (!occluded<COUNT>(sv, offset_ray2(pl.pos, n_s), dir, len - EPS, stk, c)) {
+            if (L.corrected)
+                res += r;
+            else
+                res = r;
+        }
+    }
+    return res;
+}
+
+// one merge candidate (vcm_commons.glsl:268-290); false if rejected
+LT_DEV bool merge_one(const VcmLaunch& L, const LVtx& v, float photon_count, int depth, f3 n_s, f3 wo,
+                      const Mat& mat, bool side, const VState& cs, const Payload& pl, float norm, f3& out) {
+    const f3 pp = pl.pos - v.pos;
+    const float dist_sqr = dot(pp, pp);
+    if (dist_sqr > L.radius * L.radius) return false;
+    const uint32_t dd = v.path_len + (uint32_t)depth - 1u;
+    if (dd > (uint32_t)L.max_depth) return false;
+    float pf, pr;
+    const f3 f = eval_bsdf(n_s, wo, mat, side, -v.wi, pf, pr, true);
+    if (is_zero(f)) return false;
+    const float w_light = v.d_vcm * L.eta_vc + v.d_vm * pf;
+    const float w_cam = cs.d_vcm * L.eta_vc + cs.d_vm * pr;
+    const float mis = 1.0f / (1.0f + w_light + w_cam);
+    const float cos_nrm = dot(v.n_s, n_s);
+    if (!(cos_nrm > EPS)) return false;
+    const float w = 1.0f - sqrtf(dist_sqr) / L.radius;
+    out = w * mis * photon_count * v.thr * f * cs.thr * norm * 3.0f;
+    return true;
+}
+
+// vcm_commons.glsl:251-296
+LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, const Mat& mat, bool side,
+                const VState& cs, const Payload& pl, float norm) {
+    f3 res = mk3(0.0f);
+    int lo[3], hi[3];
+    grid_idx(L, pl.pos - mk3(L.radius), lo);
+    grid_idx(L, pl.pos + mk3(L.radius), hi);
+    for (int x = lo[0]; x <= hi[0]; x++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int z = lo[2]; z <= hi[2]; z++) {
+                const uint32_t h = cell_hash(x, y, z, L.n_paths);
+                if (!L.corrected) {
+                    const uint32_t cnt = G.count[h];
+                    if (cnt == 0u) continue;
+                    const LVtx v = load_lv(G.verts + (G.key[h] - 1u));
+                    f3 r;
+                    if (merge_one(L, v, (float)(int)cnt, depth, n_s, wo, mat, side, cs, pl, norm, r)) res = r;
+                } else {
+                    for (uint32_t k = G.start[h]; k < G.start[h + 1]; k++) {
+                        const LVtx v = load_lv(G.verts + G.items[k]);
+                        int g[3];
+                        grid_idx(L, v.pos, g);
+                        if (g[0] != x || g[1] != y || g[2] != z) continue;  // another cell of the bucket
+                        f3 r;
+                        if (merge_one(L, v, 1.0f, depth, n_s, wo, mat, side, cs, pl, norm, r)) res += r;
+                    }
+                }
+            }
+    return res;
+}
+
+// vcm_eye.rgen:42-79 + vcm_commons.glsl:477-631 for pixel (x, y)
+template <bool COUNT>
+LT_DEV f3 eye_path(const SceneView& sv, const VcmLaunch& L, const Grid& G, const Camera& cam, uint32_t x, uint32_t y,
+                   const TraceStack& stk, Ctr& c) {
+    RNG seed = {x, y, L.eye_word, 0u};
+    const float screen_size = (float)L.n_paths;
+    const float r0 = rnd(seed), r1 = rnd(seed);
+    const float ux = ((float)x + 0.5f + (r0 - 0.5f)) / (float)L.width;
+    const float uy = ((float)y + 0.5f + (r1 - 0.5f)) / (float)L.height;
+    const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
+    const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
+    const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
+    VState cs;
+    cs.wi = {d4.x, d4.y, d4.z};
+    cs.pos = cam.pos;
+    cs.thr = mk3(1.0f);
+    cs.n_s = cam.nrm;
+    const float ct = fabsf(dot(cs.n_s, cs.wi));
+    cs.d_vcm = cam.area * screen_size * ct * ct * ct;
+    cs.d_vc = 0.0f;
+    cs.d_vm = 0.0f;
+    uint32_t lp = (uint32_t)(rnd(seed) * screen_size);
+    if (lp >= L.n_paths) lp = L.n_paths - 1u;  // float rounding of rand * N at very large N
+    f3 col = mk3(0.0f);
+    const float norm = 1.0f / (PI * (L.radius * L.radius) * screen_size);
+    for (int depth = 1;; depth++) {
+        Payload pl;
+        if (!closest<COUNT>(sv, cs.pos, cs.wi, pl, stk, c)) {
+            col += cs.thr * ld3(L.sky_col);
+            break;
+        }
+        f3 wo = cs.pos - pl.pos;
+        const float dist = length(pl.pos - cs.pos);
+        const float dist_sqr = dist * dist;
+        wo /= dist;
+        f3 n_s = pl.n_s, n_g = pl.n_g;
+        bool side = true;
+        if (dot(pl.n_g, wo) < 0.0f) n_g = -n_g;
+        if (dot(n_g, n_s) < 0.0f) {
+            n_s = -n_s;
+            side = false;
+        }
+        const float cos_wo = fabsf(dot(wo, n_s));
+        const Mat mat = load_material(sv, pl.material_idx, pl.uv);
+        const bool spec = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
+        cs.d_vcm *= dist_sqr;
+        cs.d_vcm /= cos_wo;
+        cs.d_vc /= cos_wo;
+        cs.d_vm /= cos_wo;
+        if (luminance(mat.emissive) > 0.0f) col += cs.thr * light_radiance(sv, L, mat, cs, pl, depth);
+        float pdf_rev = 0.0f;
+        if (!spec && depth < L.max_depth) col += connect_light<COUNT>(sv, L, n_s, wo, mat, side, cs, pl, seed, pdf_rev, stk, c);
+        if (!spec) col += connect_light_vertices<COUNT>(sv, L, G, lp, depth, n_s, wo, mat, side, cs, pdf_rev, pl, stk, c);
+        if (!spec && L.use_vm) col += merge(L, G, depth, n_s, wo, mat, side, cs, pl, norm);
+        if (depth >= L.max_depth) break;
+        float pdf_d, cos_t;
+        const float s0 = rnd(seed), s1 = rnd(seed), s2 = rnd(seed);
+        const f3 f = sample_bsdf(n_s, wo, mat, 1u, side, cs.wi, pdf_d, cos_t, s0, s1, s2);
+        const bool transm = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
+        const bool same_hem = dot(cs.wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
+        if (is_zero(f) || pdf_d == 0.0f || (!same_hem && !transm)) break;
+        pdf_rev = pdf_d;
+        if (!spec) pdf_rev = bsdf_pdf(mat, n_s, cs.wi, wo, side);
+        const float ac = fabsf(cos_t);
+        cs.pos = offset_ray(pl.pos, n_g);
+        if (!spec) {
+            cs.d_vc = (ac / pdf_d) * (L.eta_vm + cs.d_vcm + pdf_rev * cs.d_vc);
+            cs.d_vm = (ac / pdf_d) * (1.0f + cs.d_vcm * L.eta_vc + pdf_rev * cs.d_vm);
+            cs.d_vcm = 1.0f / pdf_d;
+        } else {
+            cs.d_vcm = 0.0f;
+            cs.d_vc *= ac;
+            cs.d_vm *= ac;
+        }
+        cs.thr *= f * ac / pdf_d;
+        cs.n_s = n_s;
+    }
+    return col;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vcm_eye(SceneView sv, VcmLaunch L, Grid G,
+                                                         float* __restrict__ splat, float4* __restrict__ accum,
+                                                         uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+                                                         unsigned long long* __restrict__ counts) {
+    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tiles_x = (L.width + 7) / 8, tiles_y = (L.height + 7) / 8;
+    const uint32_t n_tiles = tiles_x * tiles_y;
+    const Camera cam = camera_of(L);
+    Ctr c{0, 0, 0, 0};
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work, 1u);
+        t = __shfl(t, 0);
+        if (t >= n_tiles) break;
+        const uint32_t x = (t % tiles_x) * 8 + (lane & 7), y = (t / tiles_x) * 8 + (lane >> 3);
+        if (x >= L.width || y >= L.height) continue;
+        f3 col = eye_path<COUNT>(sv, L, G, cam, x, y, stk, c);
+        if (isnan(luminance(col))) continue;  // vcm_eye.rgen:80-83: the splat stays for the next frame
+        const size_t i = (size_t)y * L.width + x;
+        float* sp = splat + 3 * i;
+        col += mk3(sp[0], sp[1], sp[2]);
+        sp[0] = sp[1] = sp[2] = 0.0f;
+        float4 a = accum[i];
+        if (L.frame_num > 0) {
+            const float w = 1.0f / (float)(L.frame_num + 1);
+            a.x = a.x * (1.0f - w) + col.x * w;
+            a.y = a.y * (1.0f - w) + col.y * w;
+            a.z = a.z * (1.0f - w) + col.z * w;
+        } else {
+            a.x = col.x;
+            a.y = col.y;
+            a.z = col.z;
+        }
+        a.w = 1.0f;
+        accum[i] = a;
+    }
+    if (COUNT) {
+        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, tt = c.tris;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+            n += __shfl_xor(n, o);
+            tt += __shfl_xor(tt, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&counts[0], a);
+            atomicAdd(&counts[1], b);
+            atomicAdd(&counts[2], n);
+            atomicAdd(&counts[3], tt);
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill(lt_context* ctx, uint32_t* p, size_t n) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n, 0u);
+}
+
+// host restatement of pcg4d (utils.glsl:121-133) for the per-frame words
+void pcg4d_host(uint32_t v[4]) {
+    for (int k = 0; k < 4; k++) v[k] = v[k] * 1664525u + 1013904223u;
+    v[0] += v[1] * v[3];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    v[3] += v[1] * v[2];
+    for (int k = 0; k < 4; k++) v[k] ^= v[k] >> 16;
+    v[0] += v[1] * v[3];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    v[3] += v[1] * v[2];
+}
+
+}  // namespace
+
+// Per-frame RNG words: the light pass uses random_num (vcm_light.rgen:57-58)
+// and the eye pass frame_num ^ random_num (vcm_eye.rgen:35-36); random_num
+// (host rand(), VCM.cpp:147) is a pcg4d hash of (seed, frame) here.
+void vcm_frame_words(uint32_t seed, uint32_t frame_num, uint32_t* light_word, uint32_t* eye_word) {
+    uint32_t v[4] = {seed, frame_num, 0x9E3779B9u, 0x7F4A7C15u};
+    pcg4d_host(v);
+    *light_word = v[0];
+    *eye_word = frame_num ^ v[0];
+}
+
+// VCM.cpp:140-158 (merge radius of frame frame_num, hash-grid resolution)
+void vcm_radius_grid(const float smin[3], const float smax[3], float radius_factor, float alpha, uint32_t frame_num,
+                     float* radius, int32_t grid_res[3]) {
+#pragma clang fp contract(off)
+    const float dx = smax[0] - smin[0], dy = smax[1] - smin[1], dz = smax[2] - smin[2];
+    const float R = sqrtf(dx * dx + dy * dy + dz * dz) * 0.5f;
+    float r = R * radius_factor / 100.0f;
+    r /= (float)pow((double)frame_num + 1.0, 0.5 * (1.0 - (double)alpha));
+    *radius = r;
+    const float max_comp = fmaxf(dx, fmaxf(dy, dz));
+    const int base = (int)(max_comp / r);
+    const float g[3] = {dx * (float)base / max_comp, dy * (float)base / max_comp, dz * (float)base / max_comp};
+    for (int k = 0; k < 3; k++) grid_res[k] = (int)g[k] > 1 ? (int)g[k] : 1;
+}
+
+lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t height, int32_t max_depth) {
+    b.n_paths = width * height;
+    b.max_depth = max_depth;
+    const size_t N = b.n_paths, nv = (size_t)max_depth + 1, cells = 10 * N;
+    LT_HIP(ctx, hipMalloc(&b.verts, N * nv * sizeof(VcmVertex)));
+    LT_HIP(ctx, hipMalloc(&b.cnt, N * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.key, cells * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.count, (cells + 1) * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.start, (cells + 1) * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.fill, cells * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.items, N * nv * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.splat, N * 3 * sizeof(float)));
+    LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b.scan_bytes, b.count, b.start, (int)(cells + 1),
+                                                 ctx->stream));
+    LT_HIP(ctx, hipMalloc(&b.scan_tmp, b.scan_bytes));
+    fill(ctx, (uint32_t*)b.splat, N * 3);
+    fill(ctx, b.cnt, N);
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}
+
+void vcm_free(VcmBuffers& b) {
+    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.items, b.scan_tmp, b.splat};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    b = VcmBuffers{};
+}
+
+lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const lt_vcm_config& cfg,
+                     uint32_t frame_num, uint32_t frames, uint32_t seed, float4* accum, int count,
+                     uint64_t* d_counts) {
+    if (L0.max_depth != b.max_depth || L0.n_paths != b.n_paths)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "VCM launch does not match its buffers");
+    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.bdpt_blocks_per_cu;
+    lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
+    if (st != LT_OK) return st;
+    SceneView sv = make_scene_view(ctx);
+    auto* cnt = (unsigned long long*)d_counts;
+    if (count) fill(ctx, (uint32_t*)d_counts, 8);
+    const size_t N = b.n_paths, cells = 10 * N;
+    const dim3 grid((unsigned)blocks), block(kTraceBlock);
+    const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.items};
+    for (uint32_t f = 0; f < frames; f++) {
+        VcmLaunch L = L0;
+        L.frame_num = frame_num + f;
+        vcm_frame_words(seed, L.frame_num, &L.light_word, &L.eye_word);
+        vcm_radius_grid(L.bmin, L.bmax, cfg.radius_factor, cfg.alpha, L.frame_num, &L.radius, L.grid_res);
+        const float r2 = L.radius * L.radius;
+        L.eta_vcm = PI * r2 * (float)N;
+        L.eta_vc = 1.0f / L.eta_vcm;
+        L.eta_vm = L.use_vm ? PI * r2 * (float)N : 0.0f;
+        fill(ctx, ctx->counters, 2);
+        if (L.use_vm) {
+            if (!L.corrected) fill(ctx, b.key, cells);
+            fill(ctx, b.count, cells + 1);
+        }
+        if (count)
+            hipLaunchKernelGGL((k_vcm_light<true>), grid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
+                               b.key, b.count, ctx->counters, ctx->spill, cnt);
+        else
+            hipLaunchKernelGGL((k_vcm_light<false>), grid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
+                               b.key, b.count, ctx->counters, ctx->spill, cnt);
+        if (L.use_vm && L.corrected) {
+            LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, b.scan_bytes, b.count, b.start,
+                                                         (int)(cells + 1), ctx->stream));
+            fill(ctx, b.fill, cells);
+            hipLaunchKernelGGL(k_vcm_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                               b.verts, b.cnt, b.start, b.fill, b.items);
+        }
+        if (count)
+            hipLaunchKernelGGL((k_vcm_eye<true>), grid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
+                               ctx->counters + 1, ctx->spill, cnt);
+        else
+            hipLaunchKernelGGL((k_vcm_eye<false>), grid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
+                               ctx->counters + 1, ctx->spill, cnt);
+    }
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}

@@ -411,7 +411,8 @@ static vec3 eye_path(const VcmCtx& c, uint32_t x, uint32_t y, uint32_t eye_word,
     cs.d_vc = 0.0f;
     cs.d_vm = 0.0f;
     // vcm_trace_eye
-    const uint32_t lp = (uint32_t)(rand(seed) * c.screen_size);
+    uint32_t lp = (uint32_t)(rand(seed) * c.screen_size);
+    if (lp >= c.N) lp = c.N - 1u;  // float rounding of rand * N at very large N
     vec3 col = v3(0.0f);
     const float norm = 1.0f / (PI * (c.radius * c.radius) * c.screen_size);
     for (int depth = 1;; depth++) {
