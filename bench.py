@@ -4,7 +4,10 @@ one MI355X, on the procedural terrain workload (1.0M-triangle fBm heightfield
 Russian roulette after depth 3, max depth 12).  With --integrator bdpt it
 measures the BDPT caustics workload instead (20k-triangle glass sphere on a
 diffuse plane under a 15-degree spot light, 1024x1024, subpaths of up to 8
-bounces, 64 frames).
+bounces, 64 frames); with --integrator vcm the VCM workload on the same
+scene (1,048,576 light subpaths per frame stored as photons, merge radius
+0.5% of the scene diagonal shrinking with alpha = 0.75, vertex connection
+and merging, 100 frames).
 
 A step is one 1-spp frame of the whole image (one lt_render launch).  Prints
 one JSON line with the metric, the roofline of the path kernel (algorithmic
@@ -75,6 +78,31 @@ def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float):
     }
 
 
+def cpu_baseline_vcm(scene, camera, W, H, depth, frame, target_s: float, vcm_kw: dict):
+    import oracle
+
+    log("cpu baseline: building host BVH")
+    acc = oracle.Accel(scene.desc)
+    threads = oracle.hardware_threads()
+    frames, total_s, cl_all, sh_all = 0, 0.0, 0, 0
+    while total_s < target_s and frames < 16:  # whole frames: the eye pass needs the full light pass
+        _, secs, cl, sh = acc.render_vcm(camera, W, H, depth, frame_start=frame + frames, frames=1, threads=threads,
+                                         **vcm_kw)
+        frames += 1
+        total_s += secs
+        cl_all += cl
+        sh_all += sh
+    log(f"cpu baseline: {frames} full frame(s), {threads} threads")
+    return {
+        "value": round((cl_all + sh_all) / total_s / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{frames} full {W}x{H} frame(s) {frame}..{frame + frames - 1} (light and eye passes, "
+                   f"{cl_all + sh_all} rays, {total_s:.2f} s)"),
+    }
+
+
 def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
     import numpy as np
 
@@ -130,9 +158,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--integrator", choices=["path", "bdpt"], default="path")
-    ap.add_argument("--width", type=int, default=0, help="default 1920 (path) / 1024 (bdpt)")
-    ap.add_argument("--height", type=int, default=0, help="default 1080 (path) / 1024 (bdpt)")
+    ap.add_argument("--integrator", choices=["path", "bdpt", "vcm"], default="path")
+    ap.add_argument("--width", type=int, default=0, help="default 1920 (path) / 1024 (bdpt, vcm)")
+    ap.add_argument("--height", type=int, default=0, help="default 1080 (path) / 1024 (bdpt, vcm)")
+    ap.add_argument("--estimator", choices=["corrected", "reference"], default="corrected",
+                    help="VCM estimator (the reference's shader or the corrected sums)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strategy", choices=["wavefront", "megakernel"], default="wavefront",
@@ -150,17 +180,24 @@ def main() -> int:
     import lumen_amd as la
 
     bdpt = args.integrator == "bdpt"
-    W = args.width or (1024 if bdpt else 1920)
-    H = args.height or (1024 if bdpt else 1080)
-    kind = "caustics" if bdpt else "terrain"
+    vcm = args.integrator == "vcm"
+    W = args.width or (1024 if (bdpt or vcm) else 1920)
+    H = args.height or (1024 if (bdpt or vcm) else 1080)
+    kind = "caustics" if (bdpt or vcm) else "terrain"
     log(f"generating {kind} scene")
     t0 = time.time()
-    scene = la.Scene.procedural(kind, seed=7 if bdpt else 1)
+    scene = la.Scene.procedural(kind, seed=7 if (bdpt or vcm) else 1)
     log(f"scene: {scene.num_triangles} triangles in {time.time() - t0:.1f}s")
     ctx = la.Context(0)
     n_cu, arch = ctx.device_info()
+    vcm_kw = dict(radius_factor=1.0, alpha=0.75, use_vm=True, corrected=args.estimator == "corrected")
     if bdpt:
         it = la.BDPT(scene, ctx, W, H, path_length=8)
+        it.seed = 7
+    elif vcm:
+        # r_0 = 0.5% of the diagonal = 1% of the bounding radius (half diagonal)
+        it = la.VCM(scene, ctx, W, H, path_length=8, radius_factor=1.0, alpha=0.75, enable_vm=True,
+                    estimator=args.estimator)
         it.seed = 7
     else:
         it = la.Path(scene, ctx, W, H, strategy=args.strategy)
@@ -171,6 +208,8 @@ def main() -> int:
     sky = tuple(scene.integrator_cfg().sky_col)
     if bdpt and args.steps == 32 and "--steps" not in sys.argv:
         args.steps = 64  # the caustics workload is 64 spp
+    if vcm and args.steps == 32 and "--steps" not in sys.argv:
+        args.steps = 100  # the VCM workload is 100 iterations
 
     for _ in range(args.warmup):
         it.render(spp=1)
@@ -208,17 +247,22 @@ def main() -> int:
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     bytes_per_launch = (nodes * 128 + tris * 48) / args.steps
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    workload = (f"caustics-sphere20k-{W}x{H}-bdpt-d8" if bdpt else "terrain-1M+2000-boxes-1920x1080-path-d12")
+    workload = (f"caustics-sphere20k-{W}x{H}-bdpt-d8" if bdpt else
+                f"caustics-sphere20k-{W}x{H}-vcm-{args.estimator}-d8" if vcm else
+                "terrain-1M+2000-boxes-1920x1080-path-d12")
 
     cpu = None
     if not args.no_cpu:
         if bdpt:
             cpu = cpu_baseline_bdpt(scene, it.camera, W, H, depth, first, args.cpu_seconds)
+        elif vcm:
+            cpu = cpu_baseline_vcm(scene, it.camera, W, H, depth, first, args.cpu_seconds, vcm_kw)
         else:
             cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
 
     out = {
-        "metric": "Mrays/s (BDPT, 1 GPU)" if bdpt else "Mrays/s (Path, 1080p, 1 GPU)",
+        "metric": ("Mrays/s (BDPT, 1 GPU)" if bdpt else "Mrays/s (VCM, 1 GPU)" if vcm else
+                   "Mrays/s (Path, 1080p, 1 GPU)"),
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -232,7 +276,7 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {
             "workload": workload,
-            "integrator": "bdpt" if bdpt else f"path-{args.strategy}",
+            "integrator": ("bdpt" if bdpt else f"vcm-{args.estimator}" if vcm else f"path-{args.strategy}"),
             "width": W,
             "height": H,
             "spp_per_step": 1,
@@ -250,6 +294,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(workload),
             "kernel": ("one frame's launch sequence (walk + connect per pixel chunk, resolve)" if bdpt else
+                       "one frame's launch sequence (light pass, photon grid, eye pass)" if vcm else
                        "k_path_mega" if args.strategy == "megakernel" else
                        "one frame's launch sequence (extend/shade/shadow/mis per depth)"),
             "kernel_ms_avg": round(avg_ms, 3),

@@ -34,14 +34,23 @@ struct VcmVertex {
     uint4 f;   // material_idx, path_len, side, 0
 };
 
+// A photon of the corrected estimator's bucketed list, 64 B.
+struct VcmPhoton {
+    float4 a;  // pos.xyz, d_vcm
+    float4 b;  // wi.xyz (towards the light), d_vm
+    float4 c;  // throughput.xyz, path_len (u32 bits)
+    float4 d;  // n_s.xyz, 0
+};
+
 // Device state of one VCM integrator.
 //  - verts / cnt: the stored vertices of each light path (path p holds
 //    max_depth + 1 records at p * (max_depth + 1)) and their count.
 //  - cells: 10 * W * H hash buckets (utils.glsl:62-65).  Reference estimator:
 //    key = 1 + the largest vertex reference inserted into the bucket (the
 //    deterministic "last writer") and count = insertions.  Corrected
-//    estimator: count per bucket, its exclusive scan `start`, and `items` =
-//    the vertex references bucket by bucket.
+//    estimator: count per bucket, its exclusive scan `start`, and `phot` =
+//    the photons themselves, bucket by bucket, so a gather streams each
+//    bucket's records contiguously.
 //  - splat: light-tracing splats (float3 per pixel), read and cleared by the
 //    eye pass.
 struct VcmBuffers {
@@ -51,7 +60,7 @@ struct VcmBuffers {
     uint32_t* count = nullptr;
     uint32_t* start = nullptr;
     uint32_t* fill = nullptr;
-    uint32_t* items = nullptr;
+    VcmPhoton* phot = nullptr;
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     float* splat = nullptr;

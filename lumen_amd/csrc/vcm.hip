@@ -296,26 +296,33 @@ __global__ __launch_bounds__(kTraceBlock) void k_vcm_light(SceneView sv, VcmLaun
     }
 }
 
-// corrected estimator: every photon reference into its bucket's range
+// corrected estimator: every photon into its bucket's range
 __global__ void k_vcm_scatter(VcmLaunch L, const VcmVertex* __restrict__ verts, const uint32_t* __restrict__ cnt,
                               const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
-                              uint32_t* __restrict__ items) {
+                              VcmPhoton* __restrict__ phot) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.n_paths) return;
     const uint32_t nv = (uint32_t)L.max_depth + 1u;
     for (uint32_t i = 0; i < cnt[p]; i++) {
-        const float4 a = verts[(size_t)p * nv + i].a;
+        const VcmVertex& v = verts[(size_t)p * nv + i];
+        const float4 a = v.a, b = v.b, c = v.c, e = v.e;
+        const uint4 f = v.f;
         int g[3];
         grid_idx(L, f3{a.x, a.y, a.z}, g);
         const uint32_t h = cell_hash(g[0], g[1], g[2], L.n_paths);
-        items[start[h] + atomicAdd(&fill[h], 1u)] = p * nv + i;
+        VcmPhoton& o = phot[start[h] + atomicAdd(&fill[h], 1u)];
+        o.a = a;
+        o.b = make_float4(-c.x, -c.y, -c.z, c.w);
+        o.c = make_float4(e.x, e.y, e.z, __uint_as_float(f.y));
+        o.d = make_float4(b.x, b.y, b.z, 0.0f);
     }
 }
 
 // ------------------------------------------------------------------ eye pass
 struct Grid {
     const VcmVertex* verts;
-    const uint32_t *cnt, *key, *count, *start, *items;
+    const uint32_t *cnt, *key, *count, *start;
+    const VcmPhoton* phot;
 };
 
 // vcm_commons.glsl:132-143
@@ -404,24 +411,23 @@ LT_DEV f3 connect_light_vertices(const SceneView& sv, const VcmLaunch& L, const 
     return res;
 }
 
-// one merge candidate (vcm_commons.glsl:268-290); false if rejected
-LT_DEV bool merge_one(const VcmLaunch& L, const LVtx& v, float photon_count, int depth, f3 n_s, f3 wo,
-                      const Mat& mat, bool side, const VState& cs, const Payload& pl, float norm, f3& out) {
-    const f3 pp = pl.pos - v.pos;
-    const float dist_sqr = dot(pp, pp);
-    if (dist_sqr > L.radius * L.radius) return false;
-    const uint32_t dd = v.path_len + (uint32_t)depth - 1u;
+// one merge candidate (vcm_commons.glsl:268-290) of a photon at ppos within
+// radius (dist_sqr) arriving from direction pwi; false if rejected
+LT_DEV bool merge_one(const VcmLaunch& L, float dist_sqr, f3 pwi, f3 pthr, f3 pnrm, float pd_vcm, float pd_vm,
+                      uint32_t ppath_len, float photon_count, int depth, f3 n_s, f3 wo, const Mat& mat, bool side,
+                      const VState& cs, float norm, f3& out) {
+    const uint32_t dd = ppath_len + (uint32_t)depth - 1u;
     if (dd > (uint32_t)L.max_depth) return false;
     float pf, pr;
-    const f3 f = eval_bsdf(n_s, wo, mat, side, -v.wi, pf, pr, true);
+    const f3 f = eval_bsdf(n_s, wo, mat, side, pwi, pf, pr, true);
     if (is_zero(f)) return false;
-    const float w_light = v.d_vcm * L.eta_vc + v.d_vm * pf;
+    const float w_light = pd_vcm * L.eta_vc + pd_vm * pf;
     const float w_cam = cs.d_vcm * L.eta_vc + cs.d_vm * pr;
     const float mis = 1.0f / (1.0f + w_light + w_cam);
-    const float cos_nrm = dot(v.n_s, n_s);
+    const float cos_nrm = dot(pnrm, n_s);
     if (!(cos_nrm > EPS)) return false;
     const float w = 1.0f - sqrtf(dist_sqr) / L.radius;
-    out = w * mis * photon_count * v.thr * f * cs.thr * norm * 3.0f;
+    out = w * mis * photon_count * pthr * f * cs.thr * norm * 3.0f;
     return true;
 }
 
@@ -440,16 +446,30 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
                     const uint32_t cnt = G.count[h];
                     if (cnt == 0u) continue;
                     const LVtx v = load_lv(G.verts + (G.key[h] - 1u));
+                    const f3 pp = pl.pos - v.pos;
+                    const float dist_sqr = dot(pp, pp);
+                    if (dist_sqr > L.radius * L.radius) continue;
                     f3 r;
-                    if (merge_one(L, v, (float)(int)cnt, depth, n_s, wo, mat, side, cs, pl, norm, r)) res = r;
+                    if (merge_one(L, dist_sqr, -v.wi, v.thr, v.n_s, v.d_vcm, v.d_vm, v.path_len, (float)(int)cnt, depth,
+                                  n_s, wo, mat, side, cs, norm, r))
+                        res = r;
                 } else {
-                    for (uint32_t k = G.start[h]; k < G.start[h + 1]; k++) {
-                        const LVtx v = load_lv(G.verts + G.items[k]);
+                    const uint32_t k1 = G.start[h + 1];
+                    for (uint32_t k = G.start[h]; k < k1; k++) {
+                        const VcmPhoton& ph = G.phot[k];
+                        const float4 a = ph.a;
+                        const f3 ppos = {a.x, a.y, a.z};
+                        const f3 pp = pl.pos - ppos;
+                        const float dist_sqr = dot(pp, pp);
+                        if (dist_sqr > L.radius * L.radius) continue;
                         int g[3];
-                        grid_idx(L, v.pos, g);
+                        grid_idx(L, ppos, g);
                         if (g[0] != x || g[1] != y || g[2] != z) continue;  // another cell of the bucket
+                        const float4 b = ph.b, c = ph.c, d = ph.d;
                         f3 r;
-                        if (merge_one(L, v, 1.0f, depth, n_s, wo, mat, side, cs, pl, norm, r)) res += r;
+                        if (merge_one(L, dist_sqr, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, a.w, b.w,
+                                      __float_as_uint(c.w), 1.0f, depth, n_s, wo, mat, side, cs, norm, r))
+                            res += r;
                     }
                 }
             }
@@ -654,7 +674,7 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
     LT_HIP(ctx, hipMalloc(&b.count, (cells + 1) * sizeof(uint32_t)));
     LT_HIP(ctx, hipMalloc(&b.start, (cells + 1) * sizeof(uint32_t)));
     LT_HIP(ctx, hipMalloc(&b.fill, cells * sizeof(uint32_t)));
-    LT_HIP(ctx, hipMalloc(&b.items, N * nv * sizeof(uint32_t)));
+    LT_HIP(ctx, hipMalloc(&b.phot, N * nv * sizeof(VcmPhoton)));
     LT_HIP(ctx, hipMalloc(&b.splat, N * 3 * sizeof(float)));
     LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b.scan_bytes, b.count, b.start, (int)(cells + 1),
                                                  ctx->stream));
@@ -666,7 +686,7 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
 }
 
 void vcm_free(VcmBuffers& b) {
-    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.items, b.scan_tmp, b.splat};
+    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.phot, b.scan_tmp, b.splat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = VcmBuffers{};
@@ -685,7 +705,7 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
     if (count) fill(ctx, (uint32_t*)d_counts, 8);
     const size_t N = b.n_paths, cells = 10 * N;
     const dim3 grid((unsigned)blocks), block(kTraceBlock);
-    const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.items};
+    const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.phot};
     for (uint32_t f = 0; f < frames; f++) {
         VcmLaunch L = L0;
         L.frame_num = frame_num + f;
@@ -711,7 +731,7 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
                                                          (int)(cells + 1), ctx->stream));
             fill(ctx, b.fill, cells);
             hipLaunchKernelGGL(k_vcm_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, L,
-                               b.verts, b.cnt, b.start, b.fill, b.items);
+                               b.verts, b.cnt, b.start, b.fill, b.phot);
         }
         if (count)
             hipLaunchKernelGGL((k_vcm_eye<true>), grid, block, 0, ctx->stream, sv, L, G, b.splat, accum,

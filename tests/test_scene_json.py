@@ -1,0 +1,158 @@
+"""Lumen JSON + OBJ scene loader (lt_scene_load_json, LumenScene.cpp:231-513).
+Host code only: CPU tests."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import lumen_amd as la
+from lumen_amd._abi import LtError
+
+REF_SCENES = Path("/root/reference/scenes")
+
+# A scene in the reference's format, written for these tests: a quad light
+# (two triangles as one quad face) over a floor given with negative indices,
+# a `g` group with two names, a conductor with edge tint and a principled
+# material with defaults.
+OBJ = """# test scene
+v -1 0 -1
+v 1 0 -1
+v 1 0 1
+v -1 0 1
+vn 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+o floor
+f -4/1/1 -3/2/1 -2/3/1 -1/4/1
+v -0.25 1 -0.25
+v 0.25 1 -0.25
+v 0.3 1 0.25
+v -0.25 1 0.25
+vn 0 -1 0
+g light  panel
+f 5//2 8//2 7//2 6//2
+o ball
+v 0 0.2 0
+v 0.2 0.4 0
+v 0 0.4 0.2
+f 9 10 11
+"""
+
+JSON = """{
+  "integrator": {"type": "vcm", "path_length": 7, "sky_col": [0.1, 0.2, 0.3], "enable_vm": 1,
+                 "radius_factor": 0.5},
+  "bsdfs": [
+    {"name": "Floor", "refs": ["floor"], "type": "diffuse", "albedo": [0.5, 0.6, 0.7]},
+    {"name": "Light", "refs": ["light panel"], "type": "diffuse", "emissive_factor": [4, 5, 6]},
+    {"name": "Gold", "refs": ["ball"], "type": "conductor", "roughness": 0.2,
+     "reflectivity": [0.9, 0.8, 0.4], "edge_tint": [0.95, 0.85, 0.6]},
+    {"name": "Plastic", "refs": [], "type": "principled", "roughness": 0.05}
+  ],
+  "camera": {"fov": 50, "position": [0, 0.5, 3], "dir": [0, 0, 1]},
+  "lights": [{"type": "spot", "pos": [0, 3, 0], "dir": [0, 0, 0], "L": [10, 10, 10]}],
+  "mesh_file": "scene.obj"
+}
+"""
+
+
+def _write(tmp_path, obj=OBJ, js=JSON):
+    (tmp_path / "scene.obj").write_text(obj)
+    (tmp_path / "scene.json").write_text(js)
+    return tmp_path / "scene.json"
+
+
+def _arrays(sc):
+    import ctypes as C
+    d = sc.desc
+    mats = [d.materials[i] for i in range(d.num_materials)]
+    prims = [d.prim_infos[i] for i in range(d.num_prim_infos)]
+    lights = [d.lights[i] for i in range(d.num_lights)]
+    vtx = np.ctypeslib.as_array(C.cast(d.vertices, C.POINTER(C.c_float)), shape=(d.num_vertices, 8))
+    return mats, prims, lights, vtx
+
+
+def test_loads_shapes_materials_lights(tmp_path):
+    sc = la.Scene.load_json(str(_write(tmp_path)))
+    mats, prims, lights, vtx = _arrays(sc)
+    assert len(prims) == 3 and sc.num_triangles == 5
+    assert [p.material_index for p in prims] == [0, 1, 2]
+    # floor quad: |v2 - v0|^2 == |v3 - v1|^2, so the [0,1,3],[1,2,3] split (tiny_obj_loader.h:1513-1521)
+    floor = sc.triangles()[:2]
+    assert np.allclose(floor[0], [[-1, 0, -1], [1, 0, -1], [-1, 0, 1]])
+    assert np.allclose(floor[1], [[1, 0, -1], [1, 0, 1], [-1, 0, 1]])
+    # light quad (v5, v8, v7, v6): |v7 - v5|^2 = 0.5525 > |v6 - v8|^2 = 0.5, so [0,1,3],[1,2,3]
+    lt_tris = sc.triangles()[2:4]
+    assert np.allclose(lt_tris[0], [[-0.25, 1, -0.25], [-0.25, 1, 0.25], [0.25, 1, -0.25]])
+    assert np.allclose(lt_tris[1], [[-0.25, 1, 0.25], [0.3, 1, 0.25], [0.25, 1, -0.25]])
+    # uv and normals follow the face-vertex indices
+    assert np.allclose(vtx[:3, 6:8], [[0, 0], [1, 0], [0, 1]])
+    assert np.allclose(vtx[:3, 3:6], [[0, 1, 0]] * 3)
+    assert np.allclose(vtx[12, 3:6], 0.0)  # the ball has no normals
+    # materials (LumenScene.cpp:350-475)
+    assert mats[0].bsdf_type == 1 and mats[0].bsdf_props == 9 and np.allclose(mats[0].albedo[:], [0.5, 0.6, 0.7])
+    assert np.allclose(mats[1].emissive_factor[:], [4, 5, 6]) and np.allclose(mats[1].albedo[:], 1.0)
+    r, g = np.array([0.9, 0.8, 0.4], np.float32), np.array([0.95, 0.85, 0.6], np.float32)
+    eta = g * (1 - r) / (1 + r) + (1 - g) * (1 + np.sqrt(r)) / (1 - np.sqrt(r))
+    k = np.sqrt(1 / (1 - r) * ((r * (eta + 1)) ** 2 - (eta - 1) ** 2))
+    assert mats[2].bsdf_type == 16 and mats[2].bsdf_props == 8 | 4
+    assert np.allclose(mats[2].albedo[:], eta, rtol=1e-5) and np.allclose(mats[2].k[:], k, rtol=1e-5)
+    assert mats[3].bsdf_type == 32 and mats[3].bsdf_props == 8 | 2 and mats[3].sheen_tint == pytest.approx(0.5)
+    # lights: the emissive mesh, then the JSON spot light (LumenScene.cpp:70-110)
+    assert len(lights) == 2
+    assert lights[0].light_flags == 2 | 16 and lights[0].num_triangles == 2 and lights[0].prim_mesh_idx == 1
+    assert lights[1].light_flags == 1 | 16 | 32 and np.allclose(lights[1].pos[:], [0, 3, 0])
+    assert sc.desc.total_light_triangle_cnt == 3
+    tri_area = 0.5 * (0.5 * 0.5) + 0.5 * (0.5 * 0.55)
+    assert sc.desc.total_light_area == pytest.approx(tri_area, rel=1e-6)
+    cfg = sc.integrator_cfg()
+    assert cfg.kind == 2 and cfg.path_length == 7 and cfg.enable_vm == 1
+    assert cfg.radius_factor == pytest.approx(0.5) and np.allclose(cfg.sky_col[:], [0.1, 0.2, 0.3])
+    assert np.allclose(sc.desc.scene_min[:], [-1, 0, -1]) and np.allclose(sc.desc.scene_max[:], [1, 1, 1])
+
+
+@pytest.mark.parametrize("bad,code", [
+    ("{\"integrator\": {\"type\": \"path\"}, \"bsdfs\": [], ", "LT_ERR_PARSE"),
+    ("{\"bsdfs\": [{\"type\": \"diffuse\", \"texture\": \"a.png\", \"refs\": []}], \"mesh_file\": \"scene.obj\"}",
+     "LT_ERR_PARSE"),
+    ("{\"bsdfs\": [{\"type\": \"velvet\", \"refs\": []}], \"mesh_file\": \"scene.obj\"}", "LT_ERR_PARSE"),
+    ("{\"bsdfs\": [{\"type\": \"diffuse\", \"refs\": []}], \"mesh_file\": \"missing.obj\"}", "LT_ERR_PARSE"),
+])
+def test_errors(tmp_path, bad, code):
+    with pytest.raises(LtError, match=code):
+        la.Scene.load_json(str(_write(tmp_path, js=bad)))
+
+
+def test_missing_file(tmp_path):
+    with pytest.raises(LtError, match="LT_ERR_INVALID_ARG"):
+        la.Scene.load_json(str(tmp_path / "nope.json"))
+
+
+@pytest.mark.skipif(not (REF_SCENES / "cornell_box" / "cornell_box_bdpt.json").exists(),
+                    reason="the reference's scene files are not present")
+def test_reference_cornell_box():
+    """The reference's own Cornell box (scenes/cornell_box): 8 named shapes,
+    72 triangles + 1,920 quads, the light mesh as the only emitter, camera and
+    integrator block of the JSON."""
+    sc = la.Scene.load_json(str(REF_SCENES / "cornell_box" / "cornell_box_bdpt.json"))
+    mats, prims, lights, _ = _arrays(sc)
+    assert len(prims) == 8
+    assert sc.num_triangles == 72 + 2 * 1920
+    assert len(lights) == 1 and np.allclose(lights[0].L[:], [17, 12, 4])
+    light_mesh = prims[lights[0].prim_mesh_idx]
+    assert np.allclose(mats[light_mesh.material_index].emissive_factor[:], [17, 12, 4])
+    cfg = sc.integrator_cfg()
+    assert cfg.kind == 1 and cfg.path_length == 6 and np.allclose(cfg.sky_col[:], 1.0)
+    cam = sc.camera(64, 64)
+    inv_view = np.array(cam.inv_view).reshape(4, 4).T
+    assert np.allclose(inv_view[:3, 3], [0.7, 0.5, 17.5], atol=1e-5)
+
+
+@pytest.mark.skipif(not (REF_SCENES / "caustics.json").exists(), reason="the reference's scene files are not present")
+def test_reference_caustics_scene():
+    sc = la.Scene.load_json(str(REF_SCENES / "caustics.json"))
+    cfg = sc.integrator_cfg()
+    assert cfg.kind == 2 and cfg.path_length == 10 and cfg.enable_vm == 1
+    _, _, lights, _ = _arrays(sc)
+    assert any((l.light_flags & 7) == 1 for l in lights)  # the spot light
