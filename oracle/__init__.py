@@ -43,6 +43,12 @@ class VcmParams(C.Structure):
                 ("corrected", C.c_uint32), ("sky_col", C.c_float * 3)]
 
 
+class AdjParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32), ("sky_col", C.c_float * 3), ("albedo_id", C.c_int32), ("rough_id", C.c_int32),
+                ("aw", C.c_uint32), ("ah", C.c_uint32), ("rw", C.c_uint32), ("rh", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -90,6 +96,9 @@ def lib() -> C.CDLL:
         L.ora_render_vcm.restype = C.c_double
         L.ora_vcm_grid_selfcheck.argtypes = [P, P, P, C.POINTER(VcmParams), C.c_int]
         L.ora_vcm_words.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.ora_adjoint.argtypes = [P, P, P, C.POINTER(AdjParams), C.c_uint32, P, P, P, P, P, P, P, P, C.c_int]
+        L.ora_adjoint.restype = C.c_double
+        L.ora_bsdf_eval_d.argtypes = [P, P, P, P, C.c_int, P]
         L.ora_vcm_radius_grid.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_float),
                                           C.POINTER(C.c_int32)]
         L.ora_hardware_threads.restype = C.c_int
@@ -151,6 +160,14 @@ def eval_bsdf(mat, n_s, wo, wi, mode=1, side=True, rev=True):
     lib().ora_eval_bsdf(C.byref(mat), n_s.ctypes.data, wo.ctypes.data, wi.ctypes.data, mode, int(side), int(rev),
                         f.ctypes.data, C.byref(pdf), C.byref(pdf_rev))
     return f, pdf.value, pdf_rev.value
+
+
+def bsdf_eval_d(mat, n_s, wo, wi, side=True):
+    """eval_bsdf and its albedo / roughness derivatives: (f, df/dalbedo, df/droughness)."""
+    n_s, wo, wi = _f3(n_s), _f3(wo), _f3(wi)
+    out = np.zeros(9, np.float32)
+    lib().ora_bsdf_eval_d(C.byref(mat), n_s.ctypes.data, wo.ctypes.data, wi.ctypes.data, int(side), out.ctypes.data)
+    return out[0:3], out[3:6], out[6:9]
 
 
 def bsdf_pdf(mat, n_s, wo, wi, side=True) -> float:
@@ -277,6 +294,42 @@ class Accel:
         p.width, p.height, p.max_depth = width, height, max_depth
         p.seed, p.radius_factor, p.alpha = seed, radius_factor, 2.0 / 3.0
         return lib().ora_vcm_grid_selfcheck(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), nq)
+
+    def adjoint(self, camera, width, height, max_depth, spp, albedo=None, rough=None, albedo_id=-1, rough_id=-1,
+                seed=0, sky_col=(0, 0, 0), d_images=None, eval_albedo=None, eval_rough=None, threads=0):
+        """Differentiable Path estimator (no roulette) over texture variants.
+
+        albedo: float32 [V, ah, aw, 3] replacing scene texture `albedo_id`;
+        rough: float32 [V, rh, rw] replacing texture `rough_id`.  Without
+        d_images returns images [V, H, W, 3]; with d_images (same shape)
+        returns (grad_albedo, grad_rough) of sum(d_images * images) in float64.
+        eval_albedo / eval_rough (forward only): BSDF values use these texels
+        while sampling uses albedo / rough (detached-sampling estimator).
+        """
+        V = (albedo if albedo is not None else rough).shape[0]
+        p = AdjParams()
+        p.width, p.height, p.max_depth, p.spp, p.seed = width, height, max_depth, spp, seed
+        p.sky_col[:] = list(sky_col)
+        p.albedo_id, p.rough_id = albedo_id, rough_id
+        a = np.ascontiguousarray(albedo if albedo is not None else np.zeros((V, 1, 1, 3)), np.float32)
+        r = np.ascontiguousarray(rough if rough is not None else np.zeros((V, 1, 1)), np.float32)
+        p.ah, p.aw = a.shape[1:3]
+        p.rh, p.rw = r.shape[1:3]
+        thr = threads or lib().ora_hardware_threads()
+        if d_images is None:
+            img = np.zeros((V, height, width, 3), np.float32)
+            ea = None if eval_albedo is None else np.ascontiguousarray(eval_albedo, np.float32)
+            er = None if eval_rough is None else np.ascontiguousarray(eval_rough, np.float32)
+            lib().ora_adjoint(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), V, a.ctypes.data,
+                              r.ctypes.data, None if ea is None else ea.ctypes.data,
+                              None if er is None else er.ctypes.data, img.ctypes.data, None, None, None, thr)
+            return img
+        d = np.ascontiguousarray(d_images, np.float32)
+        ga = np.zeros(a.shape, np.float64)
+        gr = np.zeros(r.shape, np.float64)
+        lib().ora_adjoint(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), V, a.ctypes.data,
+                          r.ctypes.data, None, None, None, d.ctypes.data, ga.ctypes.data, gr.ctypes.data, thr)
+        return ga, gr
 
     def __del__(self):
         try:
