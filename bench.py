@@ -135,6 +135,127 @@ def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
     }
 
 
+def cpu_baseline_adjoint(scene, camera, W, H, depth, albedo, rough):
+    """The oracle's differentiable estimator and adjoint on the host cores:
+    one variant at 1 spp over the whole image, forward then backward (first
+    pass + replay), scaled to paths per second."""
+    import numpy as np
+
+    import oracle
+
+    log("cpu baseline: building host BVH")
+    acc = oracle.Accel(scene.desc)
+    threads = oracle.hardware_threads()
+    kw = dict(albedo=albedo[None], rough=rough[None], albedo_id=0, rough_id=1, seed=1)
+    t0 = time.perf_counter()
+    img = acc.adjoint(camera, W, H, depth, 1, threads=threads, **kw)
+    acc.adjoint(camera, W, H, depth, 1, threads=threads, d_images=np.ones_like(img), **kw)
+    secs = time.perf_counter() - t0
+    return {
+        "value": round(W * H / secs / 1e6, 4),
+        "unit": "Mpaths/s (forward + adjoint per path)",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the whole {W}x{H} image, 1 variant, 1 spp, forward and backward ({secs:.2f} s)",
+    }
+
+
+def bench_adjoint(args) -> int:
+    """C4: differentiable Cornell box (back wall = 16,384-triangle grid with
+    256x256 albedo and roughness textures), 512x512, 32 spp, depth 6, 32
+    texture variants per launch, L2 loss against a target render.  A step is
+    one optimisation iteration: forward (with the per-sample radiance cache),
+    loss gradient, backward."""
+    import numpy as np
+    import torch
+
+    import lumen_amd as la
+    from lumen_amd.adjoint import Adjoint
+
+    W = args.width or 512
+    H = args.height or 512
+    V, spp, depth = 32, 32, 6
+    scene = la.Scene.procedural("diff_wall", seed=5)
+    d = scene.desc
+    tex = [np.ctypeslib.as_array(d.textures[t].data, shape=(d.textures[t].height, d.textures[t].width, 4)).copy()
+           for t in range(d.num_textures)]
+    alb0, rough0 = tex[0][..., :3].astype(np.float32), tex[1][..., 0].astype(np.float32)
+    ctx = la.Context(0)
+    n_cu, arch = ctx.device_info()
+    adj = Adjoint(scene, ctx, W, H, spp=spp, variants=V, albedo_texture=0, roughness_texture=1, max_depth=depth)
+    rng = np.random.default_rng(1)
+    albs = torch.from_numpy(np.clip(alb0[None] + rng.uniform(-0.2, 0.2, (V,) + alb0.shape), 0.05, 0.95)
+                            .astype(np.float32)).cuda()
+    roughs = torch.from_numpy(np.clip(rough0[None] + rng.uniform(-0.2, 0.2, (V,) + rough0.shape), 0.1, 0.95)
+                              .astype(np.float32)).cuda()
+    # target: the base textures, rendered once
+    base_a = torch.from_numpy(np.repeat(alb0[None], V, 0)).cuda()
+    base_r = torch.from_numpy(np.repeat(rough0[None], V, 0)).cuda()
+    target, _ = adj.forward(base_a, base_r, seed=1000, cache=False)
+    torch.cuda.synchronize()
+    log(f"adjoint: {W}x{H}, {spp} spp, {V} variants, depth {depth}, cache {adj.cache_floats * 4 / 2**30:.2f} GiB")
+    steps = args.steps if "--steps" in sys.argv else 5
+    warmup = args.warmup if "--warmup" in sys.argv else 1
+
+    def step(k):
+        img, cache = adj.forward(albs, roughs, seed=k)
+        dl = 2.0 * (img - target)
+        ga, gr = adj.backward(albs, roughs, dl, seed=k, cache=cache)
+        return img, ga, gr
+
+    for k in range(warmup):
+        step(k)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        e0, e1, e2 = ev[k]
+        e0.record()
+        img, cache = adj.forward(albs, roughs, seed=warmup + k)
+        e1.record()
+        ga, gr = adj.backward(albs, roughs, 2.0 * (img - target), seed=warmup + k, cache=cache)
+        e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / steps
+    bwd_ms = sum(b.elapsed_time(c) for _, b, c in ev) / steps
+    loss = float(((img - target) ** 2).sum())
+    paths = V * W * H * spp
+    ms = wall / steps * 1e3
+    log(f"timed {steps} iterations in {wall:.3f}s: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, loss {loss:.4g}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_adjoint(scene, adj.camera, W, H, depth, alb0, rough0)
+    out = {
+        "metric": "adjoint ms/iter (C4, 1 GPU)",
+        "value": round(ms, 2),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene and texture variants)",
+        "config": {
+            "workload": f"diff-cornell-wall16k-tex256-{W}x{H}-spp{spp}-v{V}-d{depth}",
+            "variants": V, "spp": spp, "width": W, "height": H, "max_depth": depth,
+            "paths_per_iter": paths,
+            "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+            "mpaths_per_s_forward_plus_replay": round(2 * paths / (ms * 1e-3) / 1e6, 1),
+        },
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    adj.close()
+    ctx.close()
+    return 0
+
+
 def stream_copy(mib: int) -> int:
     """Copies mib MiB device-to-device 5 times with the plain copy kernel;
     reports read+write GB/s (the attainable HBM bandwidth)."""
@@ -158,7 +279,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--integrator", choices=["path", "bdpt", "vcm"], default="path")
+    ap.add_argument("--integrator", choices=["path", "bdpt", "vcm", "adjoint"], default="path")
     ap.add_argument("--width", type=int, default=0, help="default 1920 (path) / 1024 (bdpt, vcm)")
     ap.add_argument("--height", type=int, default=0, help="default 1080 (path) / 1024 (bdpt, vcm)")
     ap.add_argument("--estimator", choices=["corrected", "reference"], default="corrected",
@@ -176,6 +297,9 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.integrator == "adjoint":
+        return bench_adjoint(args)
 
     import lumen_amd as la
 

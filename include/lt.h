@@ -384,6 +384,54 @@ lt_status lt_accum_reset(lt_integrator* it);
 /* Copies output_tex (RGBA32F, row-major: texel (x,y) at 4*(y*width+x)) to host. */
 lt_status lt_readback_rgba32f(lt_integrator* it, float* host, size_t n_floats);
 
+/* ---- differentiable rendering ---------------------------------------------------- */
+
+/* No reference counterpart: the reference renders but does not
+ * differentiate.  The estimator is the Path integrator (path.rgen:26-104,
+ * pt_commons.glsl:3-42) with direct lighting on and Russian roulette removed,
+ * averaging spp samples per pixel; sample k of pixel (x, y) uses the RNG frame
+ * word k ^ seed, so with max_depth <= 5 one sample equals one Path frame.
+ * Per call, `variants` sets of texels replace the scene textures
+ * albedo_texture (rgb) and roughness_texture (first channel).  Gradients are
+ * path-replay backpropagation with detached sampling, for Lambertian albedo
+ * and for the principled BSDF with metallic, spec_trans, specular_tint,
+ * clearcoat, flatness and anisotropy zero; lt_adjoint_create refuses
+ * (LT_ERR_UNSUPPORTED) a scene where another material reads either texture. */
+typedef struct lt_adjoint lt_adjoint;
+typedef struct lt_adjoint_config {
+    uint32_t width;
+    uint32_t height;
+    int32_t max_depth;          /* path_length (1..64)                                   */
+    uint32_t spp;               /* samples per pixel per call                            */
+    uint32_t variants;          /* texel sets per call (>= 1)                            */
+    int32_t albedo_texture;     /* scene texture id replaced by the variants (-1: none)  */
+    int32_t roughness_texture;  /* scene texture id replaced by the variants (-1: none)  */
+    float sky_col[3];
+    uint32_t reserved[8];
+} lt_adjoint_config;
+/* Requires lt_scene_upload and lt_accel_build on ctx. */
+lt_status lt_adjoint_create(lt_context* ctx, const lt_adjoint_config* cfg, lt_adjoint** out);
+lt_status lt_adjoint_destroy(lt_adjoint* adj);
+/* Texel grids of the two replaced textures (0 x 0 for a texture set to -1). */
+lt_status lt_adjoint_texture_size(lt_adjoint* adj, uint32_t* albedo_w, uint32_t* albedo_h, uint32_t* rough_w,
+                                  uint32_t* rough_h);
+/* Floats of the optional per-sample radiance cache (spp * 3 * items). */
+lt_status lt_adjoint_cache_floats(lt_adjoint* adj, size_t* n);
+/* All pointers are device pointers, float32, row-major:
+ *   albedo [V][ah][aw][3], roughness [V][rh][rw], images [V][H][W][3].
+ * sample_cache (may be NULL): filled by the forward, read by a backward with
+ * the same camera, seed and texels, which then traces each path once instead
+ * of twice.  stream: the hipStream_t the kernels run on, in order with the
+ * caller's other work on it (NULL: HIP's null stream, e.g. torch's default
+ * stream); the calls are asynchronous. */
+lt_status lt_adjoint_forward(lt_adjoint* adj, const lt_camera* camera, uint32_t seed, const float* albedo,
+                             const float* roughness, float* images, float* sample_cache, void* stream);
+/* Gradients of sum(d_images * images) with respect to the texels; grad_albedo
+ * and grad_roughness (shaped like albedo and roughness) are overwritten. */
+lt_status lt_adjoint_backward(lt_adjoint* adj, const lt_camera* camera, uint32_t seed, const float* albedo,
+                              const float* roughness, const float* d_images, const float* sample_cache,
+                              float* grad_albedo, float* grad_roughness, void* stream);
+
 /* ---- metrics ------------------------------------------------------------------- */
 
 /* Squared-error sum between two device or host RGBA32F images over rgb:

@@ -105,6 +105,12 @@ class lt_vcm_config(C.Structure):
                 ("estimator", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
+class lt_adjoint_config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("spp", C.c_uint32),
+                ("variants", C.c_uint32), ("albedo_texture", C.c_int32), ("roughness_texture", C.c_int32),
+                ("sky_col", C.c_float * 3), ("reserved", C.c_uint32 * 8)]
+
+
 class lt_frame_params(C.Structure):
     _fields_ = [("camera", lt_camera), ("frame_num", C.c_uint32), ("seed", C.c_uint32), ("spp", C.c_uint32),
                 ("reserved", C.c_uint32 * 5)]
@@ -165,6 +171,13 @@ _PROTOS = {
     "lt_render_set_counting": (C.c_int, [P, C.c_int]),
     "lt_accum_reset": (C.c_int, [P]),
     "lt_readback_rgba32f": (C.c_int, [P, P, C.c_size_t]),
+    "lt_adjoint_create": (C.c_int, [P, C.POINTER(lt_adjoint_config), C.POINTER(P)]),
+    "lt_adjoint_destroy": (C.c_int, [P]),
+    "lt_adjoint_texture_size": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint32)]),
+    "lt_adjoint_cache_floats": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+    "lt_adjoint_forward": (C.c_int, [P, C.POINTER(lt_camera), C.c_uint32, P, P, P, P, P]),
+    "lt_adjoint_backward": (C.c_int, [P, C.POINTER(lt_camera), C.c_uint32, P, P, P, P, P, P, P]),
     "lt_image_sq_error": (C.c_int, [P, P, P, C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),
     "lt_scene_procedural": (C.c_int, [C.c_int, C.POINTER(lt_procedural_params), C.POINTER(P)]),
     "lt_scene_load_json": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),

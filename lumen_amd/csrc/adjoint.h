@@ -1,0 +1,45 @@
+// adjoint.h — launch parameters of the path-replay adjoint kernels.
+#pragma once
+#include "context.h"
+
+// The differentiable Path estimator: path.rgen:26-104 without roulette,
+// direct lighting on, spp samples per pixel (sample k uses the RNG frame
+// word k ^ seed), over `variants` texel sets that replace the scene textures
+// aid (albedo, rgb) and rid (roughness).
+//
+// Work items: item i covers variant i % variants of pixel p = i / variants,
+// pixels ordered by 8x8 tiles (tile p / 64, row-major inside), so one wave
+// holds the variants of few pixels.  Those lanes trace the same rays until a
+// glossy sample on a variant surface separates them, and their gradient
+// scatters go to different variants' arrays.
+struct AdjLaunch {
+    float inv_view[16];
+    float inv_projection[16];
+    uint32_t width, height;
+    int32_t max_depth;
+    uint32_t spp, seed, variants;
+    float sky_col[3];
+    int32_t aid, rid;          // replaced scene textures (-1: none)
+    uint32_t aw, ah, rw, rh;   // their texel grids
+    uint32_t tiles_x;
+    uint32_t n_items;          // tiles * 64 * variants
+};
+
+// Device arrays of one call (all row-major, float32):
+//   albedo [V][ah][aw][3], rough [V][rh][rw]       variant texels
+//   images [V][H][W][3]                            forward output
+//   cache  [spp][3][n_items]                        per-sample radiance (optional)
+//   d_images [V][H][W][3]                          adjoint of images
+//   g_albedo, g_rough like albedo, rough           texel gradients (overwritten)
+struct AdjBuffers {
+    const float* albedo;
+    const float* rough;
+    float* images;
+    float* cache;
+    const float* d_images;
+    float* g_albedo;
+    float* g_rough;
+};
+
+lt_status launch_adjoint(lt_context* ctx, const AdjLaunch& L, const AdjBuffers& B, bool backward, uint32_t* work,
+                         uint32_t* spill, uint32_t blocks, hipStream_t stream);

@@ -213,6 +213,10 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
     d.dir_light_idx = s->dir_light_idx;
     memcpy(d.smin, s->scene_min, 12);
     memcpy(d.smax, s->scene_max, 12);
+    d.h_mats.assign(s->materials, s->materials + s->num_materials);
+    d.h_rough.assign(s->num_materials, -1);
+    if (s->roughness_texture_ids) d.h_rough.assign(s->roughness_texture_ids, s->roughness_texture_ids + s->num_materials);
+    d.h_texs = descs;
     LT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     d.uploaded = true;
     return LT_OK;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/lt.h"
 #include "bvh_layout.h"
@@ -50,6 +51,11 @@ struct DeviceScene {
     int32_t dir_light_idx = -1;
     float total_light_area = 0.0f;
     float smin[3] = {0, 0, 0}, smax[3] = {0, 0, 0};
+    // host copies for validation of later calls (materials, roughness
+    // texture ids per material, texture sizes)
+    std::vector<lt_material> h_mats;
+    std::vector<int32_t> h_rough;
+    std::vector<TexDesc> h_texs;
     bool uploaded = false;
 };
 
