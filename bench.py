@@ -135,6 +135,77 @@ def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
     }
 
 
+def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x6A09E667, tol: float = 1e-3):
+    """Time until the running image reaches an RMSE of `tol`, under both
+    definitions (rmse/output_rmse.comp:23 and the standard one).
+
+    Ground truth: `gt_frames` frames of the same integrator with another RNG
+    seed.  Then the running image restarts from frame 0 and its squared
+    error S against the ground truth is read after 1, 2, 3, ... frames.
+    Lumen's RMSE is sqrt(S) / (3 N) (N pixels); the standard RMSE is
+    sqrt(S / (3 N)), taken relative to the ground truth's mean.  The first
+    frame count meeting each bound times the measured ms per frame is the
+    time.  Where the standard bound lies beyond the frames checked, the
+    frame count comes from the fitted sigma^2 of RMSE^2(k) = sigma^2 (1/k +
+    1/gt_frames) (the ground truth's own variance included) and is marked
+    extrapolated."""
+    import numpy as np
+
+    seed0, frame0 = it.seed, it.frame_num
+    it.seed = gt_seed
+    it.reset()
+    left = gt_frames
+    t0 = time.perf_counter()
+    while left > 0:
+        n = min(left, 64)
+        it.render(spp=n)
+        it.update()
+        left -= n
+    gt = it.output_tex
+    gt_s = time.perf_counter() - t0
+    mean = float(gt[..., :3].mean())
+    npx = gt.shape[0] * gt.shape[1]
+    it.seed = seed0
+    it.reset()
+    checks = sorted(set(list(range(1, 17)) + [24, 32, 48, 64, 96, 128]))
+    pts = []
+    lumen_k = None
+    done = 0
+    for k in checks:
+        it.render(spp=k - done)
+        it.update()
+        done = k
+        S = ctx.sq_error(it.output_tex, gt)
+        lum = np.sqrt(S) / (3 * npx)
+        std = np.sqrt(S / (3 * npx))
+        pts.append((k, lum, std))
+        if lumen_k is None and lum <= tol:
+            lumen_k = k
+    ks = np.array([p[0] for p in pts], np.float64)
+    std2 = np.array([p[2] for p in pts], np.float64) ** 2
+    basis = 1.0 / ks + 1.0 / gt_frames
+    sigma2 = float((basis * std2).sum() / (basis * basis).sum())
+    target2 = (tol * mean) ** 2
+    std_k = next((p[0] for p in pts if p[2] <= tol * mean), None)
+    extrapolated = std_k is None
+    if extrapolated:
+        # sigma^2 / k = (tol * mean)^2: the frames of a noise-free ground truth
+        std_k = round(sigma2 / target2, 1)
+    it.seed, it.frame_num = seed0, frame0
+    return {
+        "tol": tol,
+        "ground_truth": f"{gt_frames} frames, seed {gt_seed:#x} ({gt_s:.1f} s)",
+        "ms_per_frame": round(ms_per_frame, 3),
+        "lumen_rmse": {"definition": "sqrt(S)/(3N)", "frames": lumen_k,
+                       "seconds": None if lumen_k is None else round(lumen_k * ms_per_frame / 1e3, 4),
+                       "at_1_frame": float(pts[0][1])},
+        "standard_rmse_rel": {"definition": "sqrt(S/(3N)) / mean(ground truth)", "frames": std_k,
+                              "seconds": None if std_k is None else round(std_k * ms_per_frame / 1e3, 3),
+                              "extrapolated": extrapolated, "sigma2_per_frame": sigma2,
+                              "at_max_checked": {"frames": int(ks[-1]), "rmse_rel": float(pts[-1][2] / mean)}},
+    }
+
+
 def cpu_baseline_adjoint(scene, camera, W, H, depth, albedo, rough):
     """The oracle's differentiable estimator and adjoint on the host cores:
     one variant at 1 spp over the whole image, forward then backward (first
@@ -288,6 +359,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strategy", choices=["wavefront", "megakernel"], default="wavefront",
                     help="Path execution form (persistent wavefront stages or one megakernel)")
+    ap.add_argument("--gt-frames", type=int, default=-1,
+                    help="ground-truth frames of the time-to-RMSE measurement (0 skips it; default 1024 for "
+                         "path and bdpt, 128 for vcm)")
     ap.add_argument("--stream-copy", type=int, default=0, metavar="MIB",
                     help="only run the HBM copy kernel on MIB MiB (bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -375,6 +449,13 @@ def main() -> int:
                 f"caustics-sphere20k-{W}x{H}-vcm-{args.estimator}-d8" if vcm else
                 "terrain-1M+2000-boxes-1920x1080-path-d12")
 
+    gt_frames = args.gt_frames if args.gt_frames >= 0 else (128 if vcm else 1024)
+    rmse = None
+    if gt_frames > 0:
+        log(f"time to RMSE: ground truth of {gt_frames} frames")
+        rmse = time_to_rmse(it, ctx, wall / args.steps * 1e3, gt_frames)
+        log(f"time to RMSE: {rmse}")
+
     cpu = None
     if not args.no_cpu:
         if bdpt:
@@ -425,6 +506,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
         "cpu_baseline": cpu,
+        "time_to_rmse": rmse,
         "bvh_build_ms": round(bst.build_ms, 2),
     }
     print(json.dumps(out), flush=True)

@@ -21,7 +21,14 @@
 //                           sample/eval/pdf consistency, white furnace <= 1
 //   lights                : triangle sampling pdf = 1/A, spot falloff closed form
 //   Path integrator       : furnace scene closed form; analytic direct lighting of a plane
-//   BDPT / VCM            : not restated yet (parity unpinned)
+//   BDPT                  : furnace closed form (balance and power MIS), spot-lit plane closed
+//                           form, agreement with Path on a sealed scene (tests/test_oracle_bdpt.py)
+//   VCM                   : furnace closed form (both estimators, merging on/off), corrected
+//                           estimator agrees with Path, photon grid = brute-force radius search,
+//                           radius schedule closed form (tests/test_oracle_vcm.py)
+//   adjoint               : BSDF derivatives = central differences of eval_bsdf, forward =
+//                           Path frames, texel gradients = central differences of the forward
+//                           (tests/test_oracle_adjoint.py)
 #pragma once
 #include <cmath>
 #include <cstdint>
