@@ -379,6 +379,13 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* params);
 lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* stats);
 /* Enables per-ray counting in later lt_render calls (slower; for Mrays/s). */
 lt_status lt_render_set_counting(lt_integrator* it, int enable);
+/* Frames from a captured hipGraph (Path and BDPT; VCM: LT_ERR_UNSUPPORTED).
+ * No reference counterpart.  The first lt_render after enabling, or after a
+ * change of camera, seed, counting, scene, BVH or launch shapes, captures one
+ * frame's launch sequence; each frame is then one graph launch whose kernels
+ * read the frame number from device memory and whose last kernel advances
+ * it.  Images equal those of the plain launches. */
+lt_status lt_render_set_graph(lt_integrator* it, int enable);
 /* Reset of the accumulation (Path::update with updated=true, Path.cpp:61-68). */
 lt_status lt_accum_reset(lt_integrator* it);
 /* Copies output_tex (RGBA32F, row-major: texel (x,y) at 4*(y*width+x)) to host. */

@@ -283,6 +283,10 @@ class Integrator:
             return True
         return False
 
+    def set_graph(self, on: bool) -> None:
+        """Replay each frame from a captured hipGraph (Path, BDPT)."""
+        check(lib().lt_render_set_graph(self._it, int(on)), self.ctx.handle, "lt_render_set_graph")
+
     def set_counting(self, on: bool) -> None:
         check(lib().lt_render_set_counting(self._it, int(on)), self.ctx.handle, "lt_render_set_counting")
 

@@ -169,6 +169,7 @@ _PROTOS = {
     "lt_render": (C.c_int, [P, C.POINTER(lt_frame_params)]),
     "lt_render_stats_get": (C.c_int, [P, C.POINTER(lt_render_stats)]),
     "lt_render_set_counting": (C.c_int, [P, C.c_int]),
+    "lt_render_set_graph": (C.c_int, [P, C.c_int]),
     "lt_accum_reset": (C.c_int, [P]),
     "lt_readback_rgba32f": (C.c_int, [P, P, C.c_size_t]),
     "lt_adjoint_create": (C.c_int, [P, C.POINTER(lt_adjoint_config), C.POINTER(P)]),

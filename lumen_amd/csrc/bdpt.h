@@ -16,6 +16,10 @@ struct BdptLaunch {
     int32_t max_depth;
     uint32_t frame_word;  // RNG frame word: frame_num ^ seed (bdpt.rgen:36-37)
     uint32_t power;       // 1: power heuristic; 0: the reference's balance heuristic
+    // when set, the frame number is read here and the word is *frame_dev ^
+    // seed (a frame captured in a graph and replayed frame after frame)
+    const uint32_t* frame_dev;
+    uint32_t seed;
 };
 
 // Device state of one BDPT integrator.
@@ -42,5 +46,6 @@ struct BdptBuffers {
 lt_status bdpt_alloc(lt_context* ctx, BdptBuffers& b, uint32_t width, uint32_t height, int32_t max_depth);
 void bdpt_free(BdptBuffers& b);
 // Renders `frames` frames starting at frame_num into the running mean `accum`.
+// frames > 1 or frame_dev == nullptr; with frame_dev, one frame whose number is read there
 lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L, uint32_t frame_num, uint32_t frames,
                       uint32_t seed, float4* accum, int count, uint64_t* d_counts);

@@ -385,7 +385,7 @@ LT_DEV Vtx camera_vertex(const BdptLaunch& L, uint32_t x, uint32_t y, f3& dir, f
 template <bool COUNT>
 LT_DEV void bdpt_walks(const SceneView& sv, const BdptLaunch& L, const VStore& vs, uint32_t* __restrict__ hdr,
                        uint32_t x, uint32_t y, const TraceStack& stk, Ctr& c) {
-    RNG seed = {x, y, L.frame_word, 0u};
+    RNG seed = {x, y, L.frame_dev ? (*L.frame_dev ^ L.seed) : L.frame_word, 0u};
     const int D = L.max_depth;
     int n_light = 0, n_cam = 0;
     LightV0 l0{0u, 0u, 0.0f};
@@ -465,7 +465,7 @@ LT_DEV f3 bdpt_connect(const SceneView& sv, const BdptLaunch& L, const VStore& v
     const int n_light = (int)(counts & 0xFFu), n_cam = (int)(counts >> 8);
     const LightV0 l0{hdr[H_FLAGS * S + vs.slot], hdr[H_IDX * S + vs.slot],
                      __uint_as_float(hdr[H_PDFPOS * S + vs.slot])};
-    RNG seed = {x, y, L.frame_word, hdr[H_SEEDW * S + vs.slot]};
+    RNG seed = {x, y, L.frame_dev ? (*L.frame_dev ^ L.seed) : L.frame_word, hdr[H_SEEDW * S + vs.slot]};
     const bool power = L.power != 0u;
     const float screen_size = (float)(L.width * L.height);
     const int D = L.max_depth;
@@ -723,9 +723,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_bdpt_connect(SceneView sv, Bdpt
 
 // bdpt.rgen:77-90: camera estimate + splats, NaN guard, running mean
 __global__ void k_bdpt_resolve(const float4* __restrict__ fcol, float* __restrict__ splat, float4* __restrict__ accum,
-                               uint32_t n, uint32_t frame_num) {
+                               uint32_t n, uint32_t frame_arg, const uint32_t* __restrict__ frame_dev) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t frame_num = frame_dev ? *frame_dev : frame_arg;
     const float4 fc = fcol[i];
     float* sp = splat + 3 * (size_t)i;
     const f3 col = mk3(fc.x, fc.y, fc.z) + mk3(sp[0], sp[1], sp[2]);
@@ -795,6 +796,7 @@ lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L0, uin
     for (uint32_t f = 0; f < frames; f++) {
         BdptLaunch L = L0;
         L.frame_word = (frame_num + f) ^ seed;
+        L.seed = seed;
         for (uint32_t t0 = 0; t0 < tiles; t0 += b.chunk_tiles) {
             const uint32_t nt = tiles - t0 < b.chunk_tiles ? tiles - t0 : b.chunk_tiles;
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, (size_t)2, 0u);
@@ -811,7 +813,7 @@ lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L0, uin
             }
         }
         hipLaunchKernelGGL(k_bdpt_resolve, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, b.fcol, b.splat, accum,
-                           n, frame_num + f);
+                           n, frame_num + f, L0.frame_dev);
     }
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;

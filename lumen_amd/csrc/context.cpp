@@ -53,6 +53,7 @@ lt_status ensure_spill(lt_context* ctx, size_t threads) {
     if (ctx->spill) (void)hipFree(ctx->spill);
     ctx->spill = nullptr;
     ctx->spill_threads = 0;
+    ctx->generation++;
     LT_HIP(ctx, hipMalloc(&ctx->spill, threads * (ltb::kStackTotal - ltb::kLdsStack) * sizeof(uint32_t)));
     ctx->spill_threads = threads;
     return LT_OK;
@@ -175,6 +176,7 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
             return lt_fail(ctx, LT_ERR_INVALID_ARG, "texture " + std::to_string(t) + " is empty");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->generation++;
     free_scene(ctx->scene);
     free_bvh(ctx->bvh);
     DeviceScene& d = ctx->scene;
@@ -225,6 +227,7 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
 lt_status lt_accel_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats) {
     if (!ctx) return LT_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
+    ctx->generation++;
     return bvh_build(ctx, opts, stats);
 }
 
@@ -302,6 +305,7 @@ lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
     if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
     if (t->refill_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "refill_min must be in 1..64");
     if (t->refill_min) ctx->tuning.refill_min = t->refill_min;
+    ctx->generation++;
     return LT_OK;
 }
 

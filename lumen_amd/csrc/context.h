@@ -88,6 +88,10 @@ struct lt_context {
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
     lt_tuning tuning{8u, 8u, 32u, 4u, {}};
+    // bumped whenever device state that launches capture by pointer or value
+    // changes (scene upload, BVH build, spill reallocation, launch shapes):
+    // a captured frame graph older than this is stale
+    uint64_t generation = 1;
 };
 
 // helpers shared by the .cpp / .hip translation units
@@ -106,5 +110,7 @@ constexpr int kTraceBlock = 128;
 
 // builder / kernels (implemented in the .hip files)
 lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats);
+// writes `value` into the word at p (advance = 0), or increments it (advance = 1)
+lt_status launch_frame_word(lt_context* ctx, uint32_t* p, uint32_t value, int advance, hipStream_t stream);
 lt_status launch_trace(lt_context* ctx, const lt_ray* d_rays, size_t n, void* d_out, int any_hit, int count,
                        uint64_t* d_counts);

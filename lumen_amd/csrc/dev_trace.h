@@ -144,11 +144,12 @@ LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float t
 }
 
 // Processes one BVH8 node (its children's boxes, then its leaf triangles,
-// then pushes the hit internal children, nearest last).  ANY: finishes at the
-// first hit in [tmin, tmax].  Counters are only updated when COUNT.
-template <bool ANY, bool COUNT>
-LT_DEV bool trace_step(const SceneView& sv, TraceState& s, const TraceStack& stk, uint32_t& n_nodes,
-                       uint32_t& n_tris) {
+// then pushes the hit internal children, nearest last).  any: finishes at the
+// first hit in [tmin, tmax] (a per-lane flag; constant where trace_step's
+// template argument gives it).  Counters are only updated when COUNT.
+template <bool COUNT>
+LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack& stk, bool any, uint32_t& n_nodes,
+                           uint32_t& n_tris) {
     if (s.sp == 0) return true;
     const f3 o = s.o;
     const RayPre& rp = s.rp;
@@ -240,7 +241,7 @@ LT_DEV bool trace_step(const SceneView& sv, TraceState& s, const TraceStack& stk
                 if (isect_tri(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), o, rp, tmin, tmax, t, u,
                               v)) {
                     const uint32_t gid = __float_as_uint(cc.y);
-                    if (ANY) {
+                    if (any) {
                         best.t = t;
                         best.u = u;
                         best.v = v;
@@ -285,6 +286,12 @@ LT_DEV bool trace_step(const SceneView& sv, TraceState& s, const TraceStack& stk
     return sp == 0;
 }
 
+template <bool ANY, bool COUNT>
+LT_DEV bool trace_step(const SceneView& sv, TraceState& s, const TraceStack& stk, uint32_t& n_nodes,
+                       uint32_t& n_tris) {
+    return trace_step_dyn<COUNT>(sv, s, stk, ANY, n_nodes, n_tris);
+}
+
 // Streams the queries [0, n_items) of a queue through one wave: every lane
 // steps its own query one node at a time, and lanes whose query finished
 // take new ones (one atomic per refill, refills once sv.refill_min lanes are
@@ -325,6 +332,49 @@ LT_DEV void trace_stream(const SceneView& sv, const TraceStack& stk, uint32_t n_
             continue;
         }
         if (has && trace_step<ANY, COUNT>(sv, s, stk, n_nodes, n_tris)) {
+            finish(tag, s);
+            has = false;
+        }
+    }
+}
+
+// trace_stream over a queue whose queries are closest-hit or any-hit one by
+// one: load(item, o, d, tmin, tmax, any) also says which.
+template <bool COUNT, class Load, class Finish>
+LT_DEV void trace_stream_mixed(const SceneView& sv, const TraceStack& stk, uint32_t n_items, uint32_t* work, Load load,
+                               Finish finish, uint32_t& n_nodes, uint32_t& n_tris) {
+    const uint32_t lane = threadIdx.x & 63u;
+    TraceState s;
+    uint32_t tag = 0;
+    bool has = false, any = false;
+    bool exhausted = false;  // wave-uniform
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long idle = __ballot(!has);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= sv.refill_min || n_idle == 64u) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, n_idle);
+                base = __shfl(base, leader);
+                if (base + n_idle >= n_items) exhausted = true;
+                if (!has) {
+                    const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (mine < n_items) {
+                        f3 o, d;
+                        float tmin, tmax;
+                        tag = load(mine, o, d, tmin, tmax, any);
+                        trace_init(s, stk, o, d, tmin, tmax, sv.num_nodes != 0);
+                        has = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && trace_step_dyn<COUNT>(sv, s, stk, any, n_nodes, n_tris)) {
             finish(tag, s);
             has = false;
         }

@@ -31,7 +31,28 @@ struct lt_integrator {
     lt_render_stats last{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool rendered = false;
+    // frame graph (lt_render_set_graph): one frame's launch sequence whose
+    // kernels read the frame number at d_frame and whose last kernel
+    // advances it, replayed once per frame
+    int graph_on = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t* d_frame = nullptr;
+    bool graph_word_known = false;
+    uint32_t graph_word = 0;  // frame number the device word holds
+    lt_camera graph_cam{};
+    uint32_t graph_seed = 0;
+    int graph_counting = 0;
+    uint64_t graph_gen = 0;
 };
+
+static void drop_graph(lt_integrator* it) {
+    if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
+    if (it->graph) (void)hipGraphDestroy(it->graph);
+    it->gexec = nullptr;
+    it->graph = nullptr;
+    it->graph_word_known = false;
+}
 
 extern "C" {
 
@@ -120,20 +141,17 @@ lt_status lt_integrator_destroy(lt_integrator* it) {
     vcm_free(it->vb);
     if (it->e0) (void)hipEventDestroy(it->e0);
     if (it->e1) (void)hipEventDestroy(it->e1);
+    drop_graph(it);
+    if (it->d_frame) (void)hipFree(it->d_frame);
     delete it;
     return LT_OK;
 }
 
-lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
-    if (!it || !p) return LT_ERR_INVALID_ARG;
+// Launches `spp` frames from frame_num (frame_dev == nullptr), or one frame
+// whose number is read at frame_dev.
+static lt_status launch_frames(lt_integrator* it, const lt_frame_params* p, uint32_t frame_num, uint32_t spp,
+                               const uint32_t* frame_dev) {
     lt_context* ctx = it->ctx;
-    if (!ctx->bvh.built) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_render before lt_accel_build");
-    if (ctx->scene.nlights == 0 || ctx->scene.total_light_triangle_cnt == 0)
-        return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene has no lights");
-    (void)hipSetDevice(ctx->device);
-    const uint32_t spp = p->spp ? p->spp : 1u;
-    LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
-    lt_status st;
     if (it->kind == LT_INTEGRATOR_BDPT) {
         BdptLaunch L{};
         memcpy(L.view, p->camera.view, sizeof(L.view));
@@ -144,8 +162,10 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
         L.height = it->height;
         L.max_depth = it->bdpt.max_depth;
         L.power = it->bdpt.mis == LT_MIS_POWER ? 1u : 0u;
-        st = launch_bdpt(ctx, it->bb, L, p->frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
-    } else if (it->kind == LT_INTEGRATOR_VCM) {
+        L.frame_dev = frame_dev;
+        return launch_bdpt(ctx, it->bb, L, frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
+    }
+    if (it->kind == LT_INTEGRATOR_VCM) {
         VcmLaunch L{};
         memcpy(L.view, p->camera.view, sizeof(L.view));
         memcpy(L.projection, p->camera.projection, sizeof(L.projection));
@@ -160,27 +180,92 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
         memcpy(L.sky_col, it->vcm.sky_col, 12);
         L.use_vm = it->vcm.enable_vm ? 1u : 0u;
         L.corrected = it->vcm.estimator == LT_VCM_CORRECTED ? 1u : 0u;
-        st = launch_vcm(ctx, it->vb, L, it->vcm, p->frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
-    } else {
-        PathLaunch L{};
-        memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
-        memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
-        L.width = it->path.width;
-        L.height = it->path.height;
-        L.max_depth = it->path.max_depth;
-        memcpy(L.sky_col, it->path.sky_col, 12);
-        L.direct_lighting = it->path.direct_lighting;
-        L.frame_start = p->frame_num;
-        L.spp = spp;
-        L.seed = p->seed;
-        st = it->path.strategy == LT_PATH_WAVEFRONT
-                 ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
-                 : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+        return launch_vcm(ctx, it->vb, L, it->vcm, frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
     }
+    PathLaunch L{};
+    memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+    memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+    L.width = it->path.width;
+    L.height = it->path.height;
+    L.max_depth = it->path.max_depth;
+    memcpy(L.sky_col, it->path.sky_col, 12);
+    L.direct_lighting = it->path.direct_lighting;
+    L.frame_start = frame_num;
+    L.spp = spp;
+    L.seed = p->seed;
+    L.frame_dev = frame_dev;
+    return it->path.strategy == LT_PATH_WAVEFRONT
+               ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
+               : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+}
+
+// Captures one frame into it->gexec.  Buffers the launch functions would
+// (re)allocate are sized first: nothing may allocate while capturing.
+static lt_status capture_frame(lt_integrator* it, const lt_frame_params* p) {
+    lt_context* ctx = it->ctx;
+    drop_graph(it);
+    if (!it->d_frame) LT_HIP(ctx, hipMalloc(&it->d_frame, 256));
+    const size_t pblocks = (size_t)ctx->num_cus * ctx->tuning.path_blocks_per_cu;
+    const size_t bblocks = (size_t)ctx->num_cus * ctx->tuning.bdpt_blocks_per_cu;
+    lt_status st = ensure_spill(ctx, (it->kind == LT_INTEGRATOR_BDPT ? bblocks : pblocks) * kTraceBlock);
+    if (st == LT_OK && it->kind == LT_INTEGRATOR_PATH && it->path.strategy == LT_PATH_WAVEFRONT)
+        st = wave_alloc(ctx, it->wave, it->width * it->height);
     if (st != LT_OK) return st;
+    LT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    st = launch_frames(it, p, 0u, 1u, it->d_frame);
+    if (st == LT_OK) st = launch_frame_word(ctx, it->d_frame, 0u, 1, ctx->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st != LT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return lt_hip_check(ctx, e, "hipStreamEndCapture");
+    it->graph = g;
+    LT_HIP(ctx, hipGraphInstantiate(&it->gexec, g, nullptr, nullptr, 0));
+    it->graph_cam = p->camera;
+    it->graph_seed = p->seed;
+    it->graph_counting = it->counting;
+    it->graph_gen = ctx->generation;
+    return LT_OK;
+}
+
+lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
+    if (!it || !p) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = it->ctx;
+    if (!ctx->bvh.built) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_render before lt_accel_build");
+    if (ctx->scene.nlights == 0 || ctx->scene.total_light_triangle_cnt == 0)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene has no lights");
+    (void)hipSetDevice(ctx->device);
+    const uint32_t spp = p->spp ? p->spp : 1u;
+    lt_status st;
+    if (it->graph_on) {
+        const bool stale = !it->gexec || memcmp(&it->graph_cam, &p->camera, sizeof(lt_camera)) != 0 ||
+                           it->graph_seed != p->seed || it->graph_counting != it->counting ||
+                           it->graph_gen != ctx->generation;
+        if (stale && (st = capture_frame(it, p)) != LT_OK) return st;
+        LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
+        if (!it->graph_word_known || it->graph_word != p->frame_num)
+            if ((st = launch_frame_word(ctx, it->d_frame, p->frame_num, 0, ctx->stream)) != LT_OK) return st;
+        for (uint32_t f = 0; f < spp; f++) LT_HIP(ctx, hipGraphLaunch(it->gexec, ctx->stream));
+        it->graph_word_known = true;
+        it->graph_word = p->frame_num + spp;
+    } else {
+        LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
+        if ((st = launch_frames(it, p, p->frame_num, spp, nullptr)) != LT_OK) return st;
+    }
     LT_HIP(ctx, hipEventRecord(it->e1, ctx->stream));
     it->last.frames = spp;
     it->rendered = true;
+    return LT_OK;
+}
+
+lt_status lt_render_set_graph(lt_integrator* it, int enable) {
+    if (!it) return LT_ERR_INVALID_ARG;
+    if (enable && it->kind == LT_INTEGRATOR_VCM)
+        return lt_fail(it->ctx, LT_ERR_UNSUPPORTED, "VCM frames set their radius and grid on the host");
+    it->graph_on = enable ? 1 : 0;
+    if (!enable) drop_graph(it);
     return LT_OK;
 }
 

@@ -45,6 +45,20 @@ __global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ s
 
 }  // namespace
 
+namespace {
+
+__global__ void k_frame_word(uint32_t* p, uint32_t value, int advance) {
+    if (threadIdx.x == 0) *p = advance ? *p + 1u : value;
+}
+
+}  // namespace
+
+lt_status launch_frame_word(lt_context* ctx, uint32_t* p, uint32_t value, int advance, hipStream_t stream) {
+    hipLaunchKernelGGL(k_frame_word, dim3(1), dim3(64), 0, stream, p, value, advance);
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}
+
 extern "C" lt_status lt_stream_copy(lt_context* ctx, void* dst, const void* src, size_t bytes, float* ms) {
     if (!ctx || !dst || !src || (bytes & 15)) return LT_ERR_INVALID_ARG;
     const size_t n = bytes / 16;

@@ -14,6 +14,9 @@ struct PathLaunch {
     uint32_t frame_start;  // frame_num of the first frame of this launch
     uint32_t spp;          // frames accumulated by this launch
     uint32_t seed;         // XOR'ed into the RNG frame word
+    // when set, frame_start is read here instead (a frame captured in a
+    // graph and replayed frame after frame; the graph advances the word)
+    const uint32_t* frame_dev;
 };
 
 lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,

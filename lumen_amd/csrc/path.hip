@@ -168,8 +168,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLau
         if (x >= L.width || y >= L.height) continue;
         const size_t pix = (size_t)y * L.width + x;
         float4 acc = accum[pix];
+        const uint32_t frame0 = L.frame_dev ? *L.frame_dev : L.frame_start;
         for (uint32_t f = 0; f < L.spp; f++) {
-            const uint32_t frame_num = L.frame_start + f;
+            const uint32_t frame_num = frame0 + f;
             f3 col;
             if (!path_sample<COUNT>(sv, L, x, y, frame_num ^ L.seed, stk, col, c)) continue;  // path.rgen:102-104
             if (frame_num > 0) {

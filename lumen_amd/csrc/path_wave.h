@@ -30,6 +30,7 @@ struct WaveState {
     float4* ray_d;
     float4* thr;
     float4* col;
+    float4* misc;  // MIS-BSDF light terms, folded into col by the next shade or the accumulation
     uint4* meta;  // x = pcg4d counter word, y = last bounce specular
     uint4* hit;   // prim, inst (0xFFFFFFFF = miss), u, v
     uint32_t* queue[2];

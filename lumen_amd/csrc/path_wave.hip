@@ -2,14 +2,15 @@
 //
 // Per frame, a fixed sequence of persistent kernels over compacted queues
 // (no host synchronisation; queue lengths live in a device control block):
-//   generate                       camera rays for every pixel -> extend queue
-//   for depth in [0, max_depth):
-//     extend[d]   closest hit of the queued rays (traversal only)
+//   generate      camera rays for every pixel -> extend queue 0
+//   for depth d in [0, max_depth):
+//     trace[d]    closest hits of extend queue d, and (d > 0) the shadow
+//                 records (any hit: adds the NEE term if visible) and MIS
+//                 records (closest hit: adds the BSDF-sampled light term if it
+//                 hits the sampled light triangle) of depth d - 1, as one
+//                 work range
 //     shade[d]    emission, light sample (shadow record), MIS-BSDF record for
 //                 area lights, BSDF sample + Russian roulette -> extend queue d+1
-//     shadow[d]   any hit of the shadow records, adds the NEE term if visible
-//     mis[d]      closest hit of the MIS records, adds the BSDF-sampled light
-//                 term if it hits the sampled light triangle
 //   accumulate    running mean into the output (path.rgen:102-112)
 // Terminated paths leave the queues (ballot + one atomic per wave), so every
 // stage runs on full waves whatever the path lengths.  Each path consumes its
@@ -30,6 +31,9 @@ namespace {
 constexpr float PATH_TMIN = 0.001f;    // path.rgen:19
 constexpr float PATH_TMAX = 10000.0f;  // path.rgen:20
 constexpr int RR_MIN_DEPTH = 3;        // path.rgen:21
+
+// frame number of frame f of the launch
+LT_DEV uint32_t frame_of(const PathLaunch& L, uint32_t f) { return (L.frame_dev ? *L.frame_dev : L.frame_start) + f; }
 
 // wave-level compaction: returns this lane's slot in `queue` (one atomic per wave)
 LT_DEV uint32_t wave_append(uint32_t* len, bool pred) {
@@ -64,10 +68,11 @@ __global__ void k_wf_reset(WaveCtl* ctl) {
 }
 
 // camera rays (path.rgen:26-38); one path per pixel
-__global__ __launch_bounds__(256) void k_wf_generate(PathLaunch L, uint32_t frame_word, WaveState S, WaveCtl* ctl) {
+__global__ __launch_bounds__(256) void k_wf_generate(PathLaunch L, uint32_t f, WaveState S, WaveCtl* ctl) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = L.width * L.height;
     if (p >= n) return;
+    const uint32_t frame_word = frame_of(L, f) ^ L.seed;
     const uint32_t x = p % L.width, y = p / L.width;
     RNG seed = {x, y, frame_word, 0u};
     const float r0 = rnd(seed), r1 = rnd(seed);
@@ -82,53 +87,17 @@ __global__ __launch_bounds__(256) void k_wf_generate(PathLaunch L, uint32_t fram
     S.ray_d[p] = make_float4(d4.x, d4.y, d4.z, 0.0f);
     S.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     S.col[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    S.misc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     S.meta[p] = make_uint4(seed.w, 0u /*last_specular*/, 0u, 0u);
     S.queue[0][p] = p;
     if (p == 0) ctl->q_len[0] = n;
 }
 
-// closest hit of the extend queue
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_wf_extend(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t depth,
-                                                           uint32_t* __restrict__ spill,
-                                                           unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
-    const uint32_t* q = S.queue[depth & 1];
-    uint32_t n_nodes = 0, n_tris = 0;
-    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
-        const uint32_t p = q[item];
-        const float4 ro = S.ray_o[p], rd = S.ray_d[p];
-        o = mk3(ro.x, ro.y, ro.z);
-        d = mk3(rd.x, rd.y, rd.z);
-        tmin = PATH_TMIN;
-        tmax = PATH_TMAX;
-        return p;
-    };
-    auto finish = [&](uint32_t p, const TraceState& s) {
-        const bool hit = s.best.gid != 0xFFFFFFFFu;
-        S.hit[p] = make_uint4(hit ? s.best.prim : 0xFFFFFFFFu, hit ? s.best.inst : 0xFFFFFFFFu,
-                              __float_as_uint(s.best.u), __float_as_uint(s.best.v));
-    };
-    trace_stream<false, COUNT>(sv, stk, ctl->q_len[depth], &ctl->work[depth][0], load, finish, n_nodes, n_tris);
-    if (COUNT) {
-        unsigned long long a = n_nodes, b = n_tris;
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o);
-            b += __shfl_xor(b, o);
-        }
-        if ((threadIdx.x & 63u) == 0) {
-            atomicAdd(&counts[2], a);
-            atomicAdd(&counts[3], b);
-        }
-    }
-}
-
 // shading of the extended paths (path.rgen:49-100, pt_commons.glsl:3-42)
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, uint32_t frame_word, WaveState S,
+__global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, uint32_t f, WaveState S,
                                                   WaveCtl* ctl, uint32_t depth, unsigned long long* __restrict__ counts) {
+    const uint32_t frame_word = frame_of(L, f) ^ L.seed;
     const uint32_t* q = S.queue[depth & 1];
     uint32_t* qn = S.queue[(depth + 1) & 1];
     uint32_t n_closest = 0, n_shadow = 0;
@@ -144,6 +113,11 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, ui
         if (COUNT) n_closest++;
         const uint4 hr = S.hit[p];
         float4 col = S.col[p];
+        const float4 mc = S.misc[p];
+        col.x += mc.x;
+        col.y += mc.y;
+        col.z += mc.z;
+        S.misc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         thr4 = S.thr[p];
         const uint4 meta = S.meta[p];
         counter = meta.x;
@@ -269,108 +243,110 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, ui
     }
 }
 
-// any hit of the shadow records (pt_commons.glsl:20-27)
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_wf_shadow(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t depth,
-                                                           uint32_t* __restrict__ spill,
-                                                           unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
-    uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
-        const float4 ro = S.shadow[item].o, rd = S.shadow[item].d;
-        o = mk3(ro.x, ro.y, ro.z);
-        d = mk3(rd.x, rd.y, rd.z);
-        tmin = 0.0f;
-        tmax = ro.w;
-        if (COUNT) n_rays++;
-        return item;
-    };
-    auto finish = [&](uint32_t item, const TraceState& s) {
-        if (s.best.gid != 0xFFFFFFFFu) return;  // occluded
-        const float4 rd = S.shadow[item].d, rc = S.shadow[item].c;
-        const uint32_t p = __float_as_uint(rd.w);
-        float4 c = S.col[p];
-        c.x += rc.x;
-        c.y += rc.y;
-        c.z += rc.z;
-        S.col[p] = c;
-    };
-    trace_stream<true, COUNT>(sv, stk, ctl->sh_len[depth], &ctl->work[depth][2], load, finish, n_nodes, n_tris);
-    if (COUNT) {
-        unsigned long long a = n_nodes, b = n_tris, s = n_rays;
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o);
-            b += __shfl_xor(b, o);
-            s += __shfl_xor(s, o);
-        }
-        if ((threadIdx.x & 63u) == 0) {
-            atomicAdd(&counts[1], s);
-            atomicAdd(&counts[2], a);
-            atomicAdd(&counts[3], b);
-        }
-    }
-}
-
-// closest hit of the MIS records (pt_commons.glsl:28-40).  On a miss the
+// One trace stage: the closest hits of the extend queue of depth ext_depth
+// (when with_ext), then the shadow records (any hit, pt_commons.glsl:20-27)
+// and the MIS records (closest hit, pt_commons.glsl:28-40) of depth
+// sec_depth (when with_sec), as one concatenated work range streamed
+// through the waves.  The three queues are ready together after a shade
+// stage, so one launch and one drain serve them.  On a MIS miss the
 // reference compares the stale payload of the current vertex (ray.rmiss:13
 // only clears material_idx), which the record carries.
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_wf_mis(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t depth,
-                                                        uint32_t* __restrict__ spill,
-                                                        unsigned long long* __restrict__ counts) {
+__global__ __launch_bounds__(kTraceBlock) void k_wf_trace(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t stage,
+                                                          uint32_t ext_depth, uint32_t with_ext, uint32_t sec_depth,
+                                                          uint32_t with_sec, uint32_t* __restrict__ spill,
+                                                          unsigned long long* __restrict__ counts) {
     __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
-    uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax) {
-        const float4 ro = S.mis[item].o, rd = S.mis[item].d;
-        o = mk3(ro.x, ro.y, ro.z);
-        d = mk3(rd.x, rd.y, rd.z);
-        tmin = PATH_TMIN;
-        tmax = PATH_TMAX;
-        if (COUNT) n_rays++;
+    const uint32_t n_ext = with_ext ? ctl->q_len[ext_depth] : 0u;
+    const uint32_t n_sh = with_sec ? ctl->sh_len[sec_depth] : 0u;
+    const uint32_t n_mis = with_sec ? ctl->mis_len[sec_depth] : 0u;
+    const uint32_t* q = S.queue[ext_depth & 1];
+    uint32_t n_nodes = 0, n_tris = 0, n_shr = 0, n_misr = 0;
+    auto load = [&](uint32_t item, f3& o, f3& d, float& tmin, float& tmax, bool& any) {
+        if (item < n_ext) {
+            const uint32_t p = q[item];
+            const float4 ro = S.ray_o[p], rd = S.ray_d[p];
+            o = mk3(ro.x, ro.y, ro.z);
+            d = mk3(rd.x, rd.y, rd.z);
+            tmin = PATH_TMIN;
+            tmax = PATH_TMAX;
+            any = false;
+        } else if (item < n_ext + n_sh) {
+            const ShadowRec& r = S.shadow[item - n_ext];
+            const float4 ro = r.o, rd = r.d;
+            o = mk3(ro.x, ro.y, ro.z);
+            d = mk3(rd.x, rd.y, rd.z);
+            tmin = 0.0f;
+            tmax = ro.w;
+            any = true;
+            if (COUNT) n_shr++;
+        } else {
+            const MisRec& r = S.mis[item - n_ext - n_sh];
+            const float4 ro = r.o, rd = r.d;
+            o = mk3(ro.x, ro.y, ro.z);
+            d = mk3(rd.x, rd.y, rd.z);
+            tmin = PATH_TMIN;
+            tmax = PATH_TMAX;
+            any = false;
+            if (COUNT) n_misr++;
+        }
         return item;
     };
     auto finish = [&](uint32_t item, const TraceState& s) {
-        const MisRec r = S.mis[item];
-        const f3 wi = mk3(r.d.x, r.d.y, r.d.z);
         const bool hit = s.best.gid != 0xFFFFFFFFu;
-        const HitRec& h = s.best;
-        uint32_t tri = r.ids.z, inst = r.ids.w;
-        f3 ppos = mk3(r.vpos.x, r.vpos.y, r.vpos.z), pns = mk3(r.sns.x, r.sns.y, r.sns.z);
-        if (hit) {
-            Payload pl;
-            fill_payload(sv, h.inst, h.prim, h.u, h.v, pl);
-            tri = pl.triangle_idx;
-            inst = pl.instance_idx;
-            ppos = pl.pos;
-            pns = pl.n_s;
-        }
-        if (tri == r.ids.x && inst == r.ids.y) {
-            const f3 pos = mk3(r.vpos.x, r.vpos.y, r.vpos.z);
-            const float wl = length(ppos - pos);
-            const float g = fabsf(dot(pns, -wi)) / (wl * wl);
-            const float mis = 1.0f / (1.0f + r.o.w / (g * r.d.w));
-            const uint32_t p = __float_as_uint(r.w.w);
+        if (item < n_ext) {
+            const uint32_t p = q[item];
+            S.hit[p] = make_uint4(hit ? s.best.prim : 0xFFFFFFFFu, hit ? s.best.inst : 0xFFFFFFFFu,
+                                  __float_as_uint(s.best.u), __float_as_uint(s.best.v));
+        } else if (item < n_ext + n_sh) {
+            if (hit) return;  // occluded
+            const ShadowRec& r = S.shadow[item - n_ext];
+            const float4 rd = r.d, rc = r.c;
+            const uint32_t p = __float_as_uint(rd.w);
             float4 c = S.col[p];
-            c.x += r.w.x * mis;
-            c.y += r.w.y * mis;
-            c.z += r.w.z * mis;
+            c.x += rc.x;
+            c.y += rc.y;
+            c.z += rc.z;
             S.col[p] = c;
+        } else {
+            const MisRec r = S.mis[item - n_ext - n_sh];
+            const f3 wi = mk3(r.d.x, r.d.y, r.d.z);
+            uint32_t tri = r.ids.z, inst = r.ids.w;
+            f3 ppos = mk3(r.vpos.x, r.vpos.y, r.vpos.z), pns = mk3(r.sns.x, r.sns.y, r.sns.z);
+            if (hit) {
+                Payload pl;
+                fill_payload(sv, s.best.inst, s.best.prim, s.best.u, s.best.v, pl);
+                tri = pl.triangle_idx;
+                inst = pl.instance_idx;
+                ppos = pl.pos;
+                pns = pl.n_s;
+            }
+            if (tri == r.ids.x && inst == r.ids.y) {
+                const f3 pos = mk3(r.vpos.x, r.vpos.y, r.vpos.z);
+                const float wl = length(ppos - pos);
+                const float g = fabsf(dot(pns, -wi)) / (wl * wl);
+                const float mis = 1.0f / (1.0f + r.o.w / (g * r.d.w));
+                // the shadow record of the same path may be finishing in
+                // another lane: the MIS term goes to its own buffer
+                const uint32_t p = __float_as_uint(r.w.w);
+                S.misc[p] = make_float4(r.w.x * mis, r.w.y * mis, r.w.z * mis, 0.0f);
+            }
         }
     };
-    trace_stream<false, COUNT>(sv, stk, ctl->mis_len[depth], &ctl->work[depth][3], load, finish, n_nodes, n_tris);
+    trace_stream_mixed<COUNT>(sv, stk, n_ext + n_sh + n_mis, &ctl->work[stage][0], load, finish, n_nodes, n_tris);
     if (COUNT) {
-        unsigned long long a = n_nodes, b = n_tris, s = n_rays;
+        unsigned long long a = n_nodes, b = n_tris, sh = n_shr, mi = n_misr;
         for (int o = 32; o > 0; o >>= 1) {
             a += __shfl_xor(a, o);
             b += __shfl_xor(b, o);
-            s += __shfl_xor(s, o);
+            sh += __shfl_xor(sh, o);
+            mi += __shfl_xor(mi, o);
         }
         if ((threadIdx.x & 63u) == 0) {
-            atomicAdd(&counts[0], s);
+            atomicAdd(&counts[0], mi);
+            atomicAdd(&counts[1], sh);
             atomicAdd(&counts[2], a);
             atomicAdd(&counts[3], b);
         }
@@ -378,11 +354,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_wf_mis(SceneView sv, WaveState 
 }
 
 // running mean with the NaN guard (path.rgen:102-112)
-__global__ __launch_bounds__(256) void k_wf_accumulate(PathLaunch L, uint32_t frame_num, WaveState S,
+__global__ __launch_bounds__(256) void k_wf_accumulate(PathLaunch L, uint32_t f, WaveState S,
                                                        float4* __restrict__ accum) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.width * L.height) return;
-    const float4 c = S.col[p];
+    const uint32_t frame_num = frame_of(L, f);
+    float4 c = S.col[p];
+    const float4 mc = S.misc[p];
+    c.x += mc.x;
+    c.y += mc.y;
+    c.z += mc.z;
     if (isnan(luminance(mk3(c.x, c.y, c.z)))) return;
     float4 acc = accum[p];
     if (frame_num > 0) {
@@ -414,6 +395,7 @@ lt_status wave_alloc(lt_context* ctx, WaveBuffers& wb, uint32_t n_paths) {
     if (e == hipSuccess) e = A((void**)&wb.S.ray_d, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.thr, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.col, n * 16);
+    if (e == hipSuccess) e = A((void**)&wb.S.misc, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.meta, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.hit, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.queue[0], n * 4);
@@ -430,7 +412,7 @@ lt_status wave_alloc(lt_context* ctx, WaveBuffers& wb, uint32_t n_paths) {
 }
 
 void wave_free(WaveBuffers& wb) {
-    void* ptrs[] = {wb.S.ray_o, wb.S.ray_d, wb.S.thr, wb.S.col, wb.S.meta, wb.S.hit,
+    void* ptrs[] = {wb.S.ray_o, wb.S.ray_d, wb.S.thr, wb.S.col, wb.S.misc, wb.S.meta, wb.S.hit,
                     wb.S.queue[0], wb.S.queue[1], wb.S.shadow, wb.S.mis, wb.ctl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -452,33 +434,27 @@ lt_status launch_path_wavefront(lt_context* ctx, WaveBuffers& wb, const PathLaun
     if (count) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, 4u);
     const uint32_t depth_n = L.max_depth > 0 ? (uint32_t)L.max_depth : 1u;
     for (uint32_t f = 0; f < L.spp; f++) {
-        const uint32_t frame_num = L.frame_start + f;
-        const uint32_t fw = frame_num ^ L.seed;
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, ctx->stream, wb.ctl);
-        hipLaunchKernelGGL(k_wf_generate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, L, fw, wb.S, wb.ctl);
+        hipLaunchKernelGGL(k_wf_generate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, L, f, wb.S, wb.ctl);
+        // stage 0: the camera rays; stage d + 1: the continuation rays of
+        // depth d + 1 with the shadow and MIS rays of depth d.  The shade
+        // stage of the last depth only adds emission (path.rgen:60-62).
         for (uint32_t d = 0; d < depth_n; d++) {
-            if (count) {
-                hipLaunchKernelGGL((k_wf_extend<true>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_wf_shade<true>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, fw, wb.S,
+            const uint32_t with_sec = d > 0 ? 1u : 0u;
+            if (count)
+                hipLaunchKernelGGL((k_wf_trace<true>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
+                                   wb.ctl, d, d, 1u, d - with_sec, with_sec, ctx->spill, cnt);
+            else
+                hipLaunchKernelGGL((k_wf_trace<false>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
+                                   wb.ctl, d, d, 1u, d - with_sec, with_sec, ctx->spill, cnt);
+            if (count)
+                hipLaunchKernelGGL((k_wf_shade<true>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, f, wb.S,
                                    wb.ctl, d, cnt);
-                hipLaunchKernelGGL((k_wf_shadow<true>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_wf_mis<true>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-            } else {
-                hipLaunchKernelGGL((k_wf_extend<false>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_wf_shade<false>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, fw, wb.S,
+            else
+                hipLaunchKernelGGL((k_wf_shade<false>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, f, wb.S,
                                    wb.ctl, d, cnt);
-                hipLaunchKernelGGL((k_wf_shadow<false>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_wf_mis<false>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
-                                   wb.ctl, d, ctx->spill, cnt);
-            }
         }
-        hipLaunchKernelGGL(k_wf_accumulate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, L, frame_num, wb.S,
-                           accum);
+        hipLaunchKernelGGL(k_wf_accumulate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, L, f, wb.S, accum);
     }
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;
