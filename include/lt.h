@@ -253,7 +253,8 @@ lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_
  * keeps the current value).  Set by the autotuner (lumen_amd/autotune.py).
  * No reference counterpart: the reference's launch shapes are the driver's. */
 typedef struct lt_tuning {
-    uint32_t path_blocks_per_cu;   /* Path megakernel, 128-thread workgroups (default 8) */
+    uint32_t path_blocks_per_cu;   /* Path megakernel and wavefront trace stages, 128-thread
+                                      workgroups (default 10)                           */
     uint32_t trace_blocks_per_cu;  /* raw queries, 128-thread workgroups (default 8)     */
     uint32_t refill_min;           /* wavefront trace stages: idle lanes that trigger a
                                       refill from the queue, 1..64 (default 32)         */

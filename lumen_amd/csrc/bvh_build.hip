@@ -457,9 +457,15 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
             rc = lt_fail(ctx, LT_ERR_HIP, "BVH collapse lost triangles");
             goto cleanup;
         }
-        if (7u * (depth - 1) + 1 > (uint32_t)ltb::kStackTotal) {
+        // traversal stack: one group entry per ancestor level; entries pack
+        // child_base << 8, so node indices must stay below 2^24
+        if (depth > (uint32_t)ltb::kStackTotal) {
             rc = lt_fail(ctx, LT_ERR_UNSUPPORTED,
                          "BVH8 depth " + std::to_string(depth) + " exceeds the traversal stack bound");
+            goto cleanup;
+        }
+        if (bvh.n_nodes >= (1u << 24)) {
+            rc = lt_fail(ctx, LT_ERR_UNSUPPORTED, "BVH8 has more than 2^24 nodes");
             goto cleanup;
         }
         {

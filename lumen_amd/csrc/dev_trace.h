@@ -117,6 +117,22 @@ LT_DEV uint32_t stk_pop(const TraceStack& s, uint32_t& sp) {
     sp--;
     return sp < (uint32_t)ltb::kLdsStack ? s.lds[sp * kTraceBlock] : s.spill[sp - ltb::kLdsStack];
 }
+LT_DEV uint32_t stk_top(const TraceStack& s, uint32_t sp) {
+    return sp - 1 < (uint32_t)ltb::kLdsStack ? s.lds[(sp - 1) * kTraceBlock] : s.spill[sp - 1 - ltb::kLdsStack];
+}
+LT_DEV void stk_set_top(const TraceStack& s, uint32_t sp, uint32_t v) {
+    if (sp - 1 < (uint32_t)ltb::kLdsStack)
+        s.lds[(sp - 1) * kTraceBlock] = v;
+    else
+        s.spill[sp - 1 - ltb::kLdsStack] = v;
+}
+
+// A stack entry is a group of sibling internal nodes still to visit:
+// (child_base << 8) | mask over their ranks (child slot c is node
+// child_base + rank(c), rank = its index among the node's internal
+// children).  One entry per tree level instead of one per child; the
+// group's nodes are visited from the highest rank down.
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 // The traversal of one query as a state machine, so that a lane can take a
 // new query as soon as its current one finishes (trace_step returns true).
@@ -126,6 +142,7 @@ struct TraceState {
     float tmin, tmax, tcur;
     HitRec best;
     uint32_t sp;
+    uint32_t next;  // node to visit next (the nearest hit child), or kNoNode
 };
 
 LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float tmin, float tmax, bool has_nodes) {
@@ -140,7 +157,8 @@ LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float t
     s.best.u = s.best.v = 0.0f;
     s.rp = prep_ray(d);
     s.sp = 0;
-    if (has_nodes) stk_push(stk, s.sp, 0u);
+    s.next = has_nodes ? 0u : kNoNode;
+    (void)stk;
 }
 
 // Processes one BVH8 node (its children's boxes, then its leaf triangles,
@@ -150,7 +168,7 @@ LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float t
 template <bool COUNT>
 LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack& stk, bool any, uint32_t& n_nodes,
                            uint32_t& n_tris) {
-    if (s.sp == 0) return true;
+    if (s.next == kNoNode && s.sp == 0) return true;
     const f3 o = s.o;
     const RayPre& rp = s.rp;
     const float tmin = s.tmin, tmax = s.tmax;
@@ -158,7 +176,19 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
     HitRec& best = s.best;
     uint32_t& sp = s.sp;
     {
-        const uint32_t ni = stk_pop(stk, sp);
+        uint32_t ni = s.next;
+        if (ni == kNoNode) {
+            const uint32_t e = stk_top(stk, sp);
+            const uint32_t ranks = e & 0xFFu;
+            const uint32_t r = 31u - (uint32_t)__builtin_clz(ranks);
+            const uint32_t left = ranks & ~(1u << r);
+            if (left)
+                stk_set_top(stk, sp, (e & ~0xFFu) | left);
+            else
+                sp--;
+            ni = (e >> 8) + r;
+        }
+        s.next = kNoNode;
         if (COUNT) n_nodes++;
         const ltb::Node8* np = sv.nodes + ni;
         const float4 w0 = *((const float4*)np + 0);
@@ -272,18 +302,23 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
                     tbest = tn_c[c];
                     nearest = c;
                 }
+            // the nearest is visited next; the others wait as one group and
+            // are visited from the highest slot down (internal children
+            // sit at child_base + rank in slot order)
             uint32_t rest = hit_int & ~(1u << nearest);
-            while (rest) {
-                const int c = __builtin_ctz(rest);
-                rest &= rest - 1;
-                const uint32_t meta = ((c < 4 ? meta_lo : meta_hi) >> ((c & 3) * 8)) & 0xFF;
-                stk_push(stk, sp, child_base + meta);
+            if (rest) {
+                uint32_t ranks = 0;
+                while (rest) {
+                    const int c = __builtin_ctz(rest);
+                    rest &= rest - 1;
+                    ranks |= 1u << __popc(imask & ((1u << c) - 1u));
+                }
+                stk_push(stk, sp, (child_base << 8) | ranks);
             }
-            const uint32_t meta = ((nearest < 4 ? meta_lo : meta_hi) >> ((nearest & 3) * 8)) & 0xFF;
-            stk_push(stk, sp, child_base + meta);
+            s.next = child_base + (uint32_t)__popc(imask & ((1u << nearest) - 1u));
         }
     }
-    return sp == 0;
+    return s.next == kNoNode && sp == 0;
 }
 
 template <bool ANY, bool COUNT>

@@ -251,8 +251,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, ui
 // stage, so one launch and one drain serve them.  On a MIS miss the
 // reference compares the stale payload of the current vertex (ray.rmiss:13
 // only clears material_idx), which the record carries.
+// 96 VGPRs: five waves per SIMD hide more of the node-fetch latency
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_wf_trace(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t stage,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_wf_trace(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t stage,
                                                           uint32_t ext_depth, uint32_t with_ext, uint32_t sec_depth,
                                                           uint32_t with_sec, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ counts) {
