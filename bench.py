@@ -36,17 +36,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profile(workload: str):
-    """HBM bytes per launch from the committed rocprof counter summary, if any."""
-    p = ROOT / "profiles" / "path_megakernel_terrain1080p_pmc.json"
-    if not p.exists():
-        return None
-    try:
-        d = json.loads(p.read_text())
-        if d.get("workload") == workload:
+def traffic_from_profile(workload: str, integrator: str):
+    """HBM bytes per launch from a committed rocprof counter summary
+    (profiles/*_pmc.json) of the same workload and integrator, if any."""
+    for p in sorted((ROOT / "profiles").glob("*_pmc.json")):
+        try:
+            d = json.loads(p.read_text())
+        except Exception:
+            continue
+        if d.get("workload") == workload and d.get("integrator") == integrator:
             return d.get("traffic_bytes_per_launch")
-    except Exception:
-        return None
     return None
 
 
@@ -135,13 +134,15 @@ def cpu_baseline(scene, camera, W, H, depth, sky, frame, target_s: float):
     }
 
 
-def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x6A09E667, tol: float = 1e-3):
+def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x6A09E667, tol: float = 1e-3,
+                 budget_s: float = 20.0):
     """Time until the running image reaches an RMSE of `tol`, under both
     definitions (rmse/output_rmse.comp:23 and the standard one).
 
     Ground truth: `gt_frames` frames of the same integrator with another RNG
     seed.  Then the running image restarts from frame 0 and its squared
-    error S against the ground truth is read after 1, 2, 3, ... frames.
+    error S against the ground truth is read after 1, 2, ..., 16, 24, 32, ...
+    up to 1024 frames, until both bounds are met or `budget_s` seconds pass.
     Lumen's RMSE is sqrt(S) / (3 N) (N pixels); the standard RMSE is
     sqrt(S / (3 N)), taken relative to the ground truth's mean.  The first
     frame count meeting each bound times the measured ms per frame is the
@@ -167,10 +168,11 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
     npx = gt.shape[0] * gt.shape[1]
     it.seed = seed0
     it.reset()
-    checks = sorted(set(list(range(1, 17)) + [24, 32, 48, 64, 96, 128]))
+    checks = sorted(set(list(range(1, 17)) + [24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024]))
     pts = []
     lumen_k = None
     done = 0
+    t_check = time.perf_counter()
     for k in checks:
         it.render(spp=k - done)
         it.update()
@@ -181,6 +183,9 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
         pts.append((k, lum, std))
         if lumen_k is None and lum <= tol:
             lumen_k = k
+        # bounded: stop once both bounds are met or after budget_s seconds
+        if (lumen_k is not None and std <= tol * mean) or time.perf_counter() - t_check > budget_s:
+            break
     ks = np.array([p[0] for p in pts], np.float64)
     std2 = np.array([p[2] for p in pts], np.float64) ** 2
     basis = 1.0 / ks + 1.0 / gt_frames
@@ -198,7 +203,8 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
         "ms_per_frame": round(ms_per_frame, 3),
         "lumen_rmse": {"definition": "sqrt(S)/(3N)", "frames": lumen_k,
                        "seconds": None if lumen_k is None else round(lumen_k * ms_per_frame / 1e3, 4),
-                       "at_1_frame": float(pts[0][1])},
+                       "at_1_frame": float(pts[0][1]),
+                       "at_max_checked": {"frames": int(ks[-1]), "rmse": float(pts[-1][1])}},
         "standard_rmse_rel": {"definition": "sqrt(S/(3N)) / mean(ground truth)", "frames": std_k,
                               "seconds": None if std_k is None else round(std_k * ms_per_frame / 1e3, 3),
                               "extrapolated": extrapolated, "sigma2_per_frame": sigma2,
@@ -465,6 +471,7 @@ def main() -> int:
         else:
             cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
 
+    out_integrator = "bdpt" if bdpt else f"vcm-{args.estimator}" if vcm else f"path-{args.strategy}"
     out = {
         "metric": ("Mrays/s (BDPT, 1 GPU)" if bdpt else "Mrays/s (VCM, 1 GPU)" if vcm else
                    "Mrays/s (Path, 1080p, 1 GPU)"),
@@ -481,7 +488,7 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {
             "workload": workload,
-            "integrator": ("bdpt" if bdpt else f"vcm-{args.estimator}" if vcm else f"path-{args.strategy}"),
+            "integrator": out_integrator,
             "width": W,
             "height": H,
             "spp_per_step": 1,
@@ -497,11 +504,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(workload),
+            "traffic": traffic_from_profile(workload, out_integrator),
             "kernel": ("one frame's launch sequence (walk + connect per pixel chunk, resolve)" if bdpt else
                        "one frame's launch sequence (light pass, photon grid, eye pass)" if vcm else
                        "k_path_mega" if args.strategy == "megakernel" else
-                       "one frame's launch sequence (extend/shade/shadow/mis per depth)"),
+                       "one frame's launch sequence (trace + shade per depth)"),
             "kernel_ms_avg": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
