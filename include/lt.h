@@ -208,7 +208,7 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* scene);
 
 /* Build options for lt_accel_build. */
 typedef struct lt_build_opts {
-    uint32_t max_leaf_tris; /* triangles per BVH8 leaf child, 1..4 (default 3) */
+    uint32_t max_leaf_tris; /* triangles per BVH8 leaf child, 1..4 (0: the default, 1) */
     uint32_t reserved[7];
 } lt_build_opts;
 

@@ -128,7 +128,7 @@ class Context:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
         self.scene = scene
 
-    def build_accel(self, max_leaf_tris: int = 3) -> _abi.lt_build_stats:
+    def build_accel(self, max_leaf_tris: int = 1) -> _abi.lt_build_stats:
         o = _abi.lt_build_opts()
         o.max_leaf_tris = max_leaf_tris
         s = _abi.lt_build_stats()

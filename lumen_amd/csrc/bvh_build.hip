@@ -350,7 +350,7 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
 
 lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats) {
     if (!ctx->scene.uploaded) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_accel_build: no scene uploaded");
-    uint32_t max_leaf = opts && opts->max_leaf_tris ? opts->max_leaf_tris : 3u;
+    uint32_t max_leaf = opts && opts->max_leaf_tris ? opts->max_leaf_tris : 1u;
     if (max_leaf < 1 || max_leaf > 4) return lt_fail(ctx, LT_ERR_INVALID_ARG, "max_leaf_tris must be in 1..4");
     hipStream_t st = ctx->stream;
     DeviceBVH& bvh = ctx->bvh;

@@ -101,9 +101,14 @@ struct HitRec {
 // and the fused multiply-add one, i.e. |error| < 4 eps (65535 |a| + |b|).
 constexpr float kSlabErr = 4.76837158e-7f;
 
+// LDS-qualified so that stack accesses compile to ds_read / ds_write rather
+// than flat accesses through the generic address space
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 struct TraceStack {
-    uint32_t* lds;    // this thread's column: entry i at lds[i * kTraceBlock]
+    lds_u32* lds;     // this thread's column: entry i at lds[i * kTraceBlock]
     uint32_t* spill;  // this thread's spill area
+    LT_DEV TraceStack(uint32_t* lds_col, uint32_t* spill_area) : lds((lds_u32*)lds_col), spill(spill_area) {}
 };
 
 LT_DEV void stk_push(const TraceStack& s, uint32_t& sp, uint32_t v) {

@@ -94,8 +94,11 @@ __global__ __launch_bounds__(256) void k_wf_generate(PathLaunch L, uint32_t f, W
 }
 
 // shading of the extended paths (path.rgen:49-100, pt_commons.glsl:3-42)
+#ifndef LT_WF_SHADE_WAVES
+#define LT_WF_SHADE_WAVES 1
+#endif
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, uint32_t f, WaveState S,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LT_WF_SHADE_WAVES))) void k_wf_shade(SceneView sv, PathLaunch L, uint32_t f, WaveState S,
                                                   WaveCtl* ctl, uint32_t depth, unsigned long long* __restrict__ counts) {
     const uint32_t frame_word = frame_of(L, f) ^ L.seed;
     const uint32_t* q = S.queue[depth & 1];
@@ -252,8 +255,11 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView sv, PathLaunch L, ui
 // reference compares the stale payload of the current vertex (ray.rmiss:13
 // only clears material_idx), which the record carries.
 // 96 VGPRs: five waves per SIMD hide more of the node-fetch latency
+#ifndef LT_WF_TRACE_WAVES
+#define LT_WF_TRACE_WAVES 5
+#endif
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_wf_trace(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t stage,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_WF_TRACE_WAVES, LT_WF_TRACE_WAVES))) void k_wf_trace(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t stage,
                                                           uint32_t ext_depth, uint32_t with_ext, uint32_t sec_depth,
                                                           uint32_t with_sec, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ counts) {
