@@ -363,8 +363,9 @@ def main() -> int:
                     help="VCM estimator (the reference's shader or the corrected sums)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--strategy", choices=["wavefront", "megakernel"], default="wavefront",
-                    help="Path execution form (persistent wavefront stages or one megakernel)")
+    ap.add_argument("--strategy", choices=["persistent", "wavefront", "megakernel"], default="persistent",
+                    help="Path execution form: persistent wavefront stages, one persistent kernel with path "
+                         "regeneration, or the per-pixel megakernel")
     ap.add_argument("--gt-frames", type=int, default=-1,
                     help="ground-truth frames of the time-to-RMSE measurement (0 skips it; default 1024 for "
                          "path and bdpt, 128 for vcm)")
@@ -508,6 +509,7 @@ def main() -> int:
             "kernel": ("one frame's launch sequence (walk + connect per pixel chunk, resolve)" if bdpt else
                        "one frame's launch sequence (light pass, photon grid, eye pass)" if vcm else
                        "k_path_mega" if args.strategy == "megakernel" else
+                       "k_path_regen (one launch per frame)" if args.strategy == "persistent" else
                        "one frame's launch sequence (trace + shade per depth)"),
             "kernel_ms_avg": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),

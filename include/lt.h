@@ -260,7 +260,9 @@ typedef struct lt_tuning {
                                       refill from the queue, 1..64 (default 32)         */
     uint32_t bdpt_blocks_per_cu;   /* BDPT walk and connection kernels, 128-thread
                                       workgroups (default 4)                            */
-    uint32_t reserved[12];
+    uint32_t regen_min;            /* persistent Path kernel: idle lanes that trigger
+                                      taking new pixels, 1..64 (default 4)              */
+    uint32_t reserved[11];
 } lt_tuning;
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);
@@ -292,6 +294,8 @@ typedef enum lt_integrator_kind {
 /* Path execution strategies. */
 #define LT_PATH_MEGAKERNEL 0u /* one thread per pixel, whole path in one kernel */
 #define LT_PATH_WAVEFRONT 1u  /* persistent stages with compacted ray queues   */
+#define LT_PATH_PERSISTENT 2u /* persistent lanes with path regeneration: a lane
+                                 takes the next pixel as soon as its path ends */
 
 /* Path integrator config: SceneConfig/PathConfig (src/RayTracer/SceneConfig.h:15-25)
  * plus the PCPath fields set by Path::render (src/RayTracer/Path.cpp:27-38). */

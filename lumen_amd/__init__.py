@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRATOR_VCM, LT_PATH_MEGAKERNEL,
-                   LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS, LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL,
+                   LT_PATH_PERSISTENT, LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS, LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL,
                    LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
 
 __all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "VCM", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
@@ -122,7 +122,7 @@ class Context:
         t = _abi.lt_tuning()
         check(lib().lt_get_tuning(self._h, C.byref(t)), self._h, "lt_get_tuning")
         return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu,
-                "refill_min": t.refill_min, "bdpt_blocks_per_cu": t.bdpt_blocks_per_cu}
+                "refill_min": t.refill_min, "bdpt_blocks_per_cu": t.bdpt_blocks_per_cu, "regen_min": t.regen_min}
 
     def upload(self, scene: Scene) -> None:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
@@ -330,7 +330,8 @@ class Path(Integrator):
         self.path_length = path_length if path_length is not None else (icfg.path_length or 6)
         self.sky_col = sky_col if sky_col is not None else tuple(icfg.sky_col)
         self.direct_lighting = direct_lighting
-        self.strategy = {"megakernel": LT_PATH_MEGAKERNEL, "wavefront": LT_PATH_WAVEFRONT}[strategy]
+        self.strategy = {"megakernel": LT_PATH_MEGAKERNEL, "wavefront": LT_PATH_WAVEFRONT,
+                         "persistent": LT_PATH_PERSISTENT}[strategy]
 
     def _config(self):
         c = _abi.lt_path_config()

@@ -27,7 +27,7 @@ LT_LIGHT_SPOT, LT_LIGHT_AREA, LT_LIGHT_DIRECTIONAL, LT_LIGHT_POINT = 1, 2, 3, 4
 LT_LIGHT_FINITE, LT_LIGHT_DELTA = 1 << 4, 1 << 5
 LT_PTR_DEVICE = 1
 LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM = 0, 1, 2
-LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT = 0, 1
+LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT, LT_PATH_PERSISTENT = 0, 1, 2
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
@@ -129,7 +129,8 @@ class lt_procedural_params(C.Structure):
 
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 12)]
+                ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("regen_min", C.c_uint32),
+                ("reserved", C.c_uint32 * 11)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

@@ -32,6 +32,27 @@ def tune_path(integrator, candidates=(2, 4, 6, 8, 10, 12, 16), frames: int = 4) 
     return {"best": best, "ms": results}
 
 
+def tune_field(integrator, field: str, candidates, frames: int = 4) -> dict:
+    """Times `frames` 1-spp frames per value of one lt_tuning field
+    (e.g. regen_min of the persistent Path kernel, refill_min of the
+    wavefront trace stages); returns {value: avg ms} and sets the best."""
+    ctx = integrator.ctx
+    results = {}
+    for c in candidates:
+        ctx.set_tuning(**{field: c})
+        integrator.render()
+        integrator.update()
+        ms = []
+        for _ in range(frames):
+            integrator.render()
+            ms.append(integrator.stats().kernel_ms)
+            integrator.update()
+        results[c] = sum(ms) / len(ms)
+    best = min(results, key=results.get)
+    ctx.set_tuning(**{field: best})
+    return {"best": best, "ms": results}
+
+
 def load(arch: str, kernel: str, scene_class: str, table: Path = TABLE):
     try:
         return json.loads(table.read_text())[arch][kernel][scene_class]

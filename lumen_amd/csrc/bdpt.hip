@@ -667,7 +667,10 @@ LT_DEV void flush_counts(const Ctr& c, uint32_t lane, unsigned long long* counts
 // [tile0, tile0 + n_tiles) from `work`; a pixel's slot in the vertex and
 // record buffers is its position in the chunk.
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_bdpt_walk(SceneView sv, BdptLaunch L, float* __restrict__ verts,
+#ifndef LT_BDPT_WALK_WAVES
+#define LT_BDPT_WALK_WAVES 1
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_BDPT_WALK_WAVES))) void k_bdpt_walk(SceneView sv, BdptLaunch L, float* __restrict__ verts,
                                                            uint32_t* __restrict__ hdr, uint32_t slots, uint32_t tile0,
                                                            uint32_t n_tiles, uint32_t* __restrict__ work,
                                                            uint32_t* __restrict__ spill,
@@ -693,7 +696,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_bdpt_walk(SceneView sv, BdptLau
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_bdpt_connect(SceneView sv, BdptLaunch L, float* __restrict__ verts,
+#ifndef LT_BDPT_CONN_WAVES
+#define LT_BDPT_CONN_WAVES 1
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_BDPT_CONN_WAVES))) void k_bdpt_connect(SceneView sv, BdptLaunch L, float* __restrict__ verts,
                                                               const uint32_t* __restrict__ hdr, uint32_t slots,
                                                               uint32_t tile0, uint32_t n_tiles,
                                                               float4* __restrict__ fcol, float* __restrict__ splat,

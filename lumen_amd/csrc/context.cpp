@@ -298,13 +298,16 @@ lt_status lt_trace_count(lt_context* ctx, const lt_ray* rays, size_t n, int any_
 
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
     if (!ctx || !t) return LT_ERR_INVALID_ARG;
+    // validate everything before applying anything
     if (t->path_blocks_per_cu > 64 || t->trace_blocks_per_cu > 64 || t->bdpt_blocks_per_cu > 64)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "blocks per CU must be <= 64");
+    if (t->refill_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "refill_min must be in 1..64");
+    if (t->regen_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "regen_min must be in 1..64");
     if (t->path_blocks_per_cu) ctx->tuning.path_blocks_per_cu = t->path_blocks_per_cu;
     if (t->bdpt_blocks_per_cu) ctx->tuning.bdpt_blocks_per_cu = t->bdpt_blocks_per_cu;
     if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
-    if (t->refill_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "refill_min must be in 1..64");
     if (t->refill_min) ctx->tuning.refill_min = t->refill_min;
+    if (t->regen_min) ctx->tuning.regen_min = t->regen_min;
     ctx->generation++;
     return LT_OK;
 }

@@ -87,7 +87,7 @@ struct lt_context {
     size_t hit_buf_bytes = 0;
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
-    lt_tuning tuning{10u, 8u, 32u, 4u, {}};
+    lt_tuning tuning{10u, 8u, 32u, 4u, 4u, {}};
     // bumped whenever device state that launches capture by pointer or value
     // changes (scene upload, BVH build, spill reallocation, launch shapes):
     // a captured frame graph older than this is stale

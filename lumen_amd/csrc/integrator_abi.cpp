@@ -64,7 +64,8 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
     if (kind == LT_INTEGRATOR_PATH) {
         if (cfg_size != sizeof(lt_path_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
         const lt_path_config* pc = (const lt_path_config*)cfg;
-        if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT)
+        if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT &&
+            pc->strategy != LT_PATH_PERSISTENT)
             return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown path strategy");
         if (pc->strategy == LT_PATH_WAVEFRONT && pc->max_depth > kWaveMaxDepth)
             return lt_fail(ctx, LT_ERR_UNSUPPORTED, "wavefront path length is limited to 32");
@@ -194,9 +195,11 @@ static lt_status launch_frames(lt_integrator* it, const lt_frame_params* p, uint
     L.spp = spp;
     L.seed = p->seed;
     L.frame_dev = frame_dev;
-    return it->path.strategy == LT_PATH_WAVEFRONT
-               ? launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf)
-               : launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
+    if (it->path.strategy == LT_PATH_WAVEFRONT)
+        return launch_path_wavefront(ctx, it->wave, L, it->accum, it->counting, ctx->count_buf);
+    if (it->path.strategy == LT_PATH_PERSISTENT)
+        return launch_path_persistent(ctx, L, it->accum, it->counting, ctx->count_buf);
+    return launch_path_megakernel(ctx, L, it->accum, it->counting, ctx->count_buf);
 }
 
 // Captures one frame into it->gexec.  Buffers the launch functions would

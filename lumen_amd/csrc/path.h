@@ -17,7 +17,10 @@ struct PathLaunch {
     // when set, frame_start is read here instead (a frame captured in a
     // graph and replayed frame after frame; the graph advances the word)
     const uint32_t* frame_dev;
+    uint32_t regen_min;  // persistent kernel: idle lanes that trigger a refill
 };
 
+// persistent lanes with path regeneration, one launch per frame
+lt_status launch_path_persistent(lt_context* ctx, const PathLaunch& L, float4* accum, int count, uint64_t* d_counts);
 lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,
                                  uint64_t* d_counts);

@@ -149,7 +149,15 @@ LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, ui
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLaunch L, float4* __restrict__ accum,
+// Four waves per SIMD (128 VGPRs; the path state across a traversal spills to
+// scratch, which stays in cache): 6.5 ms per 1080p terrain frame against
+// 9.0 ms at the unconstrained 203 VGPRs / 2 waves.  A cap of five waves
+// (96 VGPRs, 416 B of scratch per lane) produced wrong images with no cause
+// found in this source, so it is not used.
+#ifndef LT_PATH_MEGA_WAVES
+#define LT_PATH_MEGA_WAVES 4
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_PATH_MEGA_WAVES))) void k_path_mega(SceneView sv, PathLaunch L, float4* __restrict__ accum,
                                                            uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
                                                            unsigned long long* __restrict__ counts) {
     __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
@@ -204,6 +212,168 @@ __global__ __launch_bounds__(kTraceBlock) void k_path_mega(SceneView sv, PathLau
     }
 }
 
+// Path regeneration (the persistent form): each lane runs one bounce of its
+// path per iteration and, once the path ends, takes the next pixel of the
+// frame (a ballot over the idle lanes and one atomic per refill), so paths
+// of uneven length (Russian roulette, escapes) never leave lanes idle until
+// the longest path of a tile ends.  Per path the arithmetic, the RNG stream
+// and the result are those of path_sample.
+struct PathState {
+    RNG seed;
+    f3 origin, direction, throughput, col;
+    Payload pl;  // persists across the path's traces (ray.rmiss:13)
+    uint32_t pix;
+    int depth;
+    bool last_specular;
+};
+
+LT_DEV void path_start(const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word, PathState& p) {
+    p.seed = {x, y, frame_word, 0u};
+    const float r0 = rnd(p.seed);
+    const float r1 = rnd(p.seed);
+    const float jx = r0 - 0.5f, jy = r1 - 0.5f;
+    const float ux = ((float)x + 0.5f + jx) / (float)L.width, uy = ((float)y + 0.5f + jy) / (float)L.height;
+    const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    const f4 o4 = mul4(L.inv_view, f4{0.0f, 0.0f, 0.0f, 1.0f});
+    const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
+    const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
+    const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
+    p.origin = {o4.x, o4.y, o4.z};
+    p.direction = {d4.x, d4.y, d4.z};
+    p.col = mk3(0.0f);
+    p.throughput = mk3(1.0f);
+    p.last_specular = false;
+    p.depth = 0;
+    p.pl.valid = false;
+    p.pl.triangle_idx = p.pl.instance_idx = 0xFFFFFFFFu;
+    p.pl.pos = p.pl.n_s = p.pl.n_g = mk3(0.0f);
+}
+
+// one iteration of path.rgen:46-100; returns false when the path has ended
+template <bool COUNT>
+LT_DEV bool path_bounce(const SceneView& sv, const PathLaunch& L, PathState& p, const TraceStack& stk, Ctr& c) {
+    trace_closest<COUNT>(sv, p.origin, p.direction, stk, p.pl, c);
+    if (!p.pl.valid) {
+        if (p.depth > 0 || L.direct_lighting == 1) p.col += p.throughput * ld3(L.sky_col);
+        return false;
+    }
+    const Payload hp = p.pl;
+    const Mat hit_mat = load_material(sv, hp.material_idx, hp.uv);
+    if ((p.depth == 0 && L.direct_lighting == 1) || p.last_specular) p.col += p.throughput * hit_mat.emissive;
+    if (p.depth >= L.max_depth - 1) return false;
+    const f3 wo = -p.direction;
+    f3 n_s = hp.n_s;
+    bool side = true;
+    f3 n_g = hp.n_g;
+    if (dot(hp.n_g, wo) < 0.0f) n_g = -n_g;
+    if (dot(n_g, hp.n_s) < 0.0f) {
+        n_s = -n_s;
+        side = false;
+    }
+    p.origin = offset_ray(hp.pos, n_g);
+    p.last_specular = is_specular(hit_mat);
+    if (!p.last_specular) {
+        const float light_pick_pdf = 1.0f / (float)sv.total_light_triangle_cnt;
+        if (p.depth > 0 || L.direct_lighting == 1)
+            p.col += p.throughput *
+                     uniform_sample_light<COUNT>(sv, p.seed, hit_mat, hp.pos, side, n_s, wo, p.pl, stk, c) /
+                     light_pick_pdf;
+    }
+    float pdf, cos_theta;
+    const float s0 = rnd(p.seed), s1 = rnd(p.seed), s2 = rnd(p.seed);
+    const f3 f = sample_bsdf(n_s, wo, hit_mat, 1, side, p.direction, pdf, cos_theta, s0, s1, s2);
+    if (pdf == 0.0f) return false;
+    p.throughput *= f * fabsf(cos_theta) / pdf;
+    float rr_scale = 1.0f;
+    if (hit_mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) rr_scale *= side ? 1.0f / hit_mat.ior : hit_mat.ior;
+    if (p.depth > RR_MIN_DEPTH) {
+        const float rr_prob = fminf(0.95f, luminance(p.throughput) * rr_scale);
+        if (rr_prob == 0.0f || rr_prob < rnd(p.seed)) return false;
+        p.throughput /= rr_prob;
+    }
+    p.depth++;
+    return true;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_PATH_MEGA_WAVES))) void k_path_regen(
+    SceneView sv, PathLaunch L, float4* __restrict__ accum, uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+    unsigned long long* __restrict__ counts) {
+    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tiles_x = (L.width + 7) / 8, tiles_y = (L.height + 7) / 8;
+    const uint32_t n_items = tiles_x * tiles_y * 64u;  // pixels in 8x8-tile order
+    const uint32_t frame_num = L.frame_dev ? *L.frame_dev : L.frame_start;
+    const uint32_t frame_word = frame_num ^ L.seed;
+    Ctr c{0, 0, 0, 0};
+    PathState p;
+    bool has = false, exhausted = false;  // exhausted is wave-uniform
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long idle = __ballot(!has);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= L.regen_min || n_idle == 64u) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, n_idle);
+                base = __shfl(base, leader);
+                if (base + n_idle >= n_items) exhausted = true;
+                if (!has) {
+                    const uint32_t t = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (t < n_items) {
+                        const uint32_t tile = t >> 6, w = t & 63u;
+                        const uint32_t x = (tile % tiles_x) * 8 + (w & 7u), y = (tile / tiles_x) * 8 + (w >> 3);
+                        if (x < L.width && y < L.height) {
+                            path_start(L, x, y, frame_word, p);
+                            p.pix = y * L.width + x;
+                            has = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && !path_bounce<COUNT>(sv, L, p, stk, c)) {
+            has = false;
+            if (!isnan(luminance(p.col))) {  // path.rgen:102-112
+                float4 acc = accum[p.pix];
+                if (frame_num > 0) {
+                    const float w = 1.0f / (float)(frame_num + 1);
+                    acc.x = acc.x * (1.0f - w) + p.col.x * w;
+                    acc.y = acc.y * (1.0f - w) + p.col.y * w;
+                    acc.z = acc.z * (1.0f - w) + p.col.z * w;
+                } else {
+                    acc.x = p.col.x;
+                    acc.y = p.col.y;
+                    acc.z = p.col.z;
+                }
+                acc.w = 1.0f;
+                accum[p.pix] = acc;
+            }
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, t = c.tris;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+            n += __shfl_xor(n, o);
+            t += __shfl_xor(t, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&counts[0], a);
+            atomicAdd(&counts[1], b);
+            atomicAdd(&counts[2], n);
+            atomicAdd(&counts[3], t);
+        }
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -226,6 +396,32 @@ lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* a
     else
         hipLaunchKernelGGL((k_path_mega<false>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
                            ctx->counters, ctx->spill, cnt);
+    LT_HIP(ctx, hipGetLastError());
+    return LT_OK;
+}
+
+lt_status launch_path_persistent(lt_context* ctx, const PathLaunch& L0, float4* accum, int count,
+                                 uint64_t* d_counts) {
+    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.path_blocks_per_cu;
+    lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
+    if (st != LT_OK) return st;
+    if (L0.frame_dev && L0.spp != 1) return lt_fail(ctx, LT_ERR_INVALID_ARG, "a captured frame renders one frame");
+    SceneView sv = make_scene_view(ctx);
+    auto* cnt = (unsigned long long*)d_counts;
+    if (count) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, 8u);
+    for (uint32_t f = 0; f < L0.spp; f++) {  // the running mean needs the frames in order
+        PathLaunch L = L0;
+        L.frame_start = L0.frame_start + f;
+        L.spp = 1;
+        L.regen_min = ctx->tuning.regen_min ? ctx->tuning.regen_min : 4u;
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 4u);
+        if (count)
+            hipLaunchKernelGGL((k_path_regen<true>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
+                               ctx->counters, ctx->spill, cnt);
+        else
+            hipLaunchKernelGGL((k_path_regen<false>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
+                               ctx->counters, ctx->spill, cnt);
+    }
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;
 }

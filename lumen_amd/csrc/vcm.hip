@@ -558,7 +558,10 @@ LT_DEV f3 eye_path(const SceneView& sv, const VcmLaunch& L, const Grid& G, const
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vcm_eye(SceneView sv, VcmLaunch L, Grid G,
+#ifndef LT_VCM_EYE_WAVES
+#define LT_VCM_EYE_WAVES 2
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_EYE_WAVES, LT_VCM_EYE_WAVES))) void k_vcm_eye(SceneView sv, VcmLaunch L, Grid G,
                                                          float* __restrict__ splat, float4* __restrict__ accum,
                                                          uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
                                                          unsigned long long* __restrict__ counts) {

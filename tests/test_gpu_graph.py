@@ -38,6 +38,7 @@ def _check(a, b, exact, tag):
 @pytest.mark.parametrize("kind,kw,exact", [
     ("path", dict(strategy="megakernel"), True),
     ("path", dict(strategy="wavefront"), True),
+    ("path", dict(strategy="persistent"), True),
     ("bdpt", dict(path_length=5), False),
 ])
 def test_graph_frames_equal_eager_frames(gpu_ctx, kind, kw, exact):

@@ -71,14 +71,16 @@ def test_running_mean_equals_single_launch(gpu_ctx):
     ("terrain", 96, 54, 2, dict(terrain_tris=20000, num_instances=100, num_point_lights=8)),
     ("caustics", 48, 40, 2, {}),
 ])
-def test_wavefront_equals_megakernel(gpu_ctx, kind, W, H, frames, kw):
-    """Same per-path random streams and decisions: the two execution forms differ
-    only in the float summation order of the per-vertex terms."""
-    print(f"[gpu] wavefront vs megakernel {kind}", flush=True)
+@pytest.mark.parametrize("strategy", ["wavefront", "persistent"])
+def test_execution_forms_equal_megakernel(gpu_ctx, kind, W, H, frames, kw, strategy):
+    """Same per-path random streams and decisions: the execution forms differ
+    only in the float summation order of the per-vertex terms (wavefront) or
+    in which lane runs a path (persistent, path regeneration)."""
+    print(f"[gpu] {strategy} vs megakernel {kind}", flush=True)
     sc = la.Scene.procedural(kind, seed=1, **kw)
     imgs = []
-    for strategy in ("megakernel", "wavefront"):
-        it = la.Path(sc, gpu_ctx, W, H, strategy=strategy)
+    for strat in ("megakernel", strategy):
+        it = la.Path(sc, gpu_ctx, W, H, strategy=strat)
         it.init()
         it.create_accel()
         it.render(spp=frames)
