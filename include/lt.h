@@ -258,11 +258,16 @@ typedef struct lt_tuning {
     uint32_t trace_blocks_per_cu;  /* raw queries, 128-thread workgroups (default 8)     */
     uint32_t refill_min;           /* wavefront trace stages: idle lanes that trigger a
                                       refill from the queue, 1..64 (default 32)         */
-    uint32_t bdpt_blocks_per_cu;   /* BDPT walk and connection kernels, 128-thread
-                                      workgroups (default 4)                            */
-    uint32_t regen_min;            /* persistent Path kernel: idle lanes that trigger
-                                      taking new pixels, 1..64 (default 4)              */
-    uint32_t reserved[11];
+    uint32_t bdpt_blocks_per_cu;   /* BDPT and VCM persistent kernels, 128-thread
+                                      workgroups; default 0 = each kernel's resident
+                                      workgroups per CU (its register and LDS use).
+                                      lt_set_tuning leaves a 0 field unchanged.         */
+    uint32_t regen_min;            /* persistent Path kernel and BDPT walk: idle lanes
+                                      that trigger taking new pixels, 1..64 (default 4) */
+    uint32_t bdpt_chunk_tiles;     /* BDPT: 8x8 pixel tiles per walk/connection pass,
+                                      read when an integrator is created; default 0 =
+                                      as many as fit a 4 GiB vertex store               */
+    uint32_t reserved[10];
 } lt_tuning;
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);

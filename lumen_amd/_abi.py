@@ -130,7 +130,7 @@ class lt_procedural_params(C.Structure):
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("regen_min", C.c_uint32),
-                ("reserved", C.c_uint32 * 11)]
+                ("bdpt_chunk_tiles", C.c_uint32), ("reserved", C.c_uint32 * 10)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

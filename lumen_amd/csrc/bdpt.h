@@ -3,7 +3,9 @@
 #include "context.h"
 
 constexpr int kBdptMaxDepth = 16;           // subpaths hold at most max_depth + 1 vertices
-constexpr uint32_t kBdptChunkTiles = 4096;  // 8x8 tiles per walk/connect pass (262,144 pixels)
+// Vertex-store budget of one walk/connect pass: a pass covers as many 8x8
+// tiles as fit (1024x1024 at max_depth 8 is one pass of 1.3 GB).
+constexpr size_t kBdptChunkBytes = (size_t)4 << 30;
 
 // PCBDPT (src/shaders/integrators/bdpt/bdpt_commons.h:5-16) plus the
 // SceneUBO camera matrices the ray generation and the splat projection read.
@@ -20,6 +22,7 @@ struct BdptLaunch {
     // seed (a frame captured in a graph and replayed frame after frame)
     const uint32_t* frame_dev;
     uint32_t seed;
+    uint32_t regen_min;  // walk kernel: idle lanes that trigger taking new pixels
 };
 
 // Device state of one BDPT integrator.
@@ -45,6 +48,8 @@ struct BdptBuffers {
 
 lt_status bdpt_alloc(lt_context* ctx, BdptBuffers& b, uint32_t width, uint32_t height, int32_t max_depth);
 void bdpt_free(BdptBuffers& b);
+// threads of the largest BDPT grid (the traversal spill buffer it needs)
+size_t bdpt_spill_threads(lt_context* ctx);
 // Renders `frames` frames starting at frame_num into the running mean `accum`.
 // frames > 1 or frame_dev == nullptr; with frame_dev, one frame whose number is read there
 lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L, uint32_t frame_num, uint32_t frames,

@@ -19,6 +19,8 @@
 // reference's O(s + t) walk over global memory.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bdpt.h"
 #include "dev_scene.h"
 #include "dev_trace.h"
@@ -129,76 +131,93 @@ LT_DEV f3 eval_f(const SceneView& sv, const Vtx& v, const Mat& m, f3 wo, f3 wi) 
     return eval_bsdf(v.n_s, wo, m, vside(v), wi, p, pr, false);
 }
 
-// bdpt_commons.glsl:13-197 (eye = false: the light walk, mode 0; eye = true:
-// the camera walk, mode 1).  Vertex 0 of the subpath (index -1 of the walk)
-// is given in registers; returns the number of vertices written after it.
+// Walk state of one pixel's subpaths (one lane): the light subpath, then
+// the camera subpath, one bounce per step.  The pixel is (seed.x, seed.y);
+// `st` packs the subpath (bit 0: 0 light, mode 0; 1 camera, mode 1), a
+// finite light (bit 1), the vertices written after vertex 0 (bits 8-15) and
+// the light subpath's length once it has ended (bits 16-23).  The light
+// vertex 0 fields MIS needs live in the pixel record, written at the start.
+struct WalkState {
+    RNG seed;
+    f3 ray_pos, wi, thr, prev_pos, prev_n;
+    float pdf_fwd;
+    uint32_t slot, st;
+    LT_DEV int sub() const { return (int)(st & 1u); }
+    LT_DEV bool finite() const { return (st & 2u) != 0u; }
+    LT_DEV int b() const { return (int)((st >> 8) & 0xFFu); }
+    LT_DEV void set_b(int b) { st = (st & ~0xFF00u) | ((uint32_t)b << 8); }
+    LT_DEV int n_light() const { return (int)((st >> 16) & 0xFFu); }
+};
+
+// One bounce of bdpt_commons.glsl:13-197 (random_walk) for subpath w.sub();
+// false once the subpath has ended (w.b() + 1 vertices).  pdf_pos_inf is the
+// light's pdf_pos if the light is infinite, else 0: the first light vertex
+// then takes pdf_pos * |cos| as its forward pdf (bdpt_commons.glsl:239-241,
+// applied after the walk there; nothing reads it in between).
 template <bool COUNT>
-LT_DEV int random_walk(const SceneView& sv, const VStore& vs, bool eye, int max_depth, f3 v0_pos, f3 v0_n, f3 wi,
-                       bool finite_light, f3 throughput, float pdf, RNG& seed, const TraceStack& stk, Ctr& c) {
-    const int sub = eye ? 1 : 0;
-    if (max_depth == 0) return 0;
-    int b = 0;
-    f3 ray_pos = v0_pos;
-    f3 prev_pos = v0_pos, prev_n = v0_n;
-    float pdf_fwd = pdf;
-    for (;;) {
-        if (COUNT) c.closest++;
-        HitRec h;
-        if (!trace_bvh<false, COUNT>(sv, ray_pos, wi, BDPT_TMIN, BDPT_TMAX, h, stk, c.nodes, c.tris)) {
-            if (eye) {  // the escaped vertex keeps the zeroed fields (BDPT.cpp:76-77)
-                Vtx e = zero_vtx();
-                e.thr = throughput;
-                e.pdf_fwd = pdf_fwd;
-                vs.store(sub, b + 1, e);
-                b++;
-            }
-            break;
+LT_DEV bool walk_step(const SceneView& sv, const VStore& vs, int max_depth, WalkState& w, float pdf_pos_inf,
+                      const TraceStack& stk, Ctr& c) {
+    const int sub = w.sub();
+    const bool eye = sub == 1;
+    int b = w.b();
+    if (COUNT) c.closest++;
+    HitRec h;
+    if (!trace_bvh<false, COUNT>(sv, w.ray_pos, w.wi, BDPT_TMIN, BDPT_TMAX, h, stk, c.nodes, c.tris)) {
+        if (eye) {  // the escaped vertex keeps the zeroed fields (BDPT.cpp:76-77)
+            Vtx e = zero_vtx();
+            e.thr = w.thr;
+            e.pdf_fwd = w.pdf_fwd;
+            vs.store(sub, b + 1, e);
+            w.set_b(b + 1);
         }
-        Payload p;
-        fill_payload(sv, h.inst, h.prim, h.u, h.v, p);
-        f3 wo = prev_pos - p.pos;
-        const float wo_len = length(wo);
-        wo /= wo_len;
-        f3 n_s = p.n_s;
-        bool side = true;
-        f3 n_g = p.n_g;
-        if (dot(p.n_g, wo) < 0.0f) n_g = -n_g;
-        if (dot(n_g, n_s) < 0.0f) {
-            n_s = n_s * -1.0f;
-            side = false;
-        }
-        const Mat mat = load_material(sv, p.material_idx, p.uv);
-        const bool mat_specular = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
-        const bool mat_transmissive = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
-        Vtx cur;
-        cur.pdf_fwd = pdf_fwd * fabsf(dot(wo, n_s)) / (wo_len * wo_len);
-        cur.n_s = n_s;
-        cur.area = p.area;
-        cur.pos = p.pos;
-        cur.uv = p.uv;
-        cur.mat = p.material_idx;
-        cur.thr = throughput;
-        cur.bits = (mat_specular ? 1u : 0u) | (side ? 2u : 0u);
-        cur.pdf_rev = 0.0f;
-        cur.pre = 0.0f;
-        vs.store(sub, b + 1, cur);
-        if (++b >= max_depth) break;
-        float cos_theta;
-        const float r0 = rnd(seed), r1 = rnd(seed), r2 = rnd(seed);
-        const f3 f = sample_bsdf(n_s, wo, mat, eye ? 1u : 0u, side, wi, pdf_fwd, cos_theta, r0, r1, r2);
-        const bool same_hem = dot(wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
-        if (is_zero(f) || pdf_fwd == 0.0f || (!same_hem && !mat_transmissive)) break;
-        throughput *= f * fabsf(cos_theta) / pdf_fwd;
-        float pdf_rev = pdf_fwd;
-        if (!mat_specular) pdf_rev = bsdf_pdf(mat, n_s, wi, wo, side);
-        const bool g_term = eye || b - 2 > -1 || finite_light;  // prev = b - 2 after the increment
-        if (g_term) pdf_rev *= fabsf(dot(prev_n, wo)) / (wo_len * wo_len);
-        vs.put(sub, b - 1, F_PREV, pdf_rev);
-        ray_pos = offset_ray(p.pos, n_g);
-        prev_pos = p.pos;
-        prev_n = n_s;
+        return false;
     }
-    return b;
+    Payload p;
+    fill_payload(sv, h.inst, h.prim, h.u, h.v, p);
+    f3 wo = w.prev_pos - p.pos;
+    const float wo_len = length(wo);
+    wo /= wo_len;
+    f3 n_s = p.n_s;
+    bool side = true;
+    f3 n_g = p.n_g;
+    if (dot(p.n_g, wo) < 0.0f) n_g = -n_g;
+    if (dot(n_g, n_s) < 0.0f) {
+        n_s = n_s * -1.0f;
+        side = false;
+    }
+    const Mat mat = load_material(sv, p.material_idx, p.uv);
+    const bool mat_specular = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
+    const bool mat_transmissive = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
+    Vtx cur;
+    cur.pdf_fwd = w.pdf_fwd * fabsf(dot(wo, n_s)) / (wo_len * wo_len);
+    if (!eye && b == 0 && !w.finite()) cur.pdf_fwd = pdf_pos_inf * fabsf(dot(w.wi, n_s));
+    cur.n_s = n_s;
+    cur.area = p.area;
+    cur.pos = p.pos;
+    cur.uv = p.uv;
+    cur.mat = p.material_idx;
+    cur.thr = w.thr;
+    cur.bits = (mat_specular ? 1u : 0u) | (side ? 2u : 0u);
+    cur.pdf_rev = 0.0f;
+    cur.pre = 0.0f;
+    vs.store(sub, b + 1, cur);
+    w.set_b(++b);
+    if (b >= max_depth) return false;
+    float cos_theta;
+    const float r0 = rnd(w.seed), r1 = rnd(w.seed), r2 = rnd(w.seed);
+    const f3 f = sample_bsdf(n_s, wo, mat, eye ? 1u : 0u, side, w.wi, w.pdf_fwd, cos_theta, r0, r1, r2);
+    const bool same_hem = dot(w.wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
+    if (is_zero(f) || w.pdf_fwd == 0.0f || (!same_hem && !mat_transmissive)) return false;
+    w.thr *= f * fabsf(cos_theta) / w.pdf_fwd;
+    float pdf_rev = w.pdf_fwd;
+    if (!mat_specular) pdf_rev = bsdf_pdf(mat, n_s, w.wi, wo, side);
+    const bool g_term = eye || b - 2 > -1 || w.finite();  // prev = b - 2 after the increment
+    if (g_term) pdf_rev *= fabsf(dot(w.prev_n, wo)) / (wo_len * wo_len);
+    vs.put(sub, b - 1, F_PREV, pdf_rev);
+    w.ray_pos = offset_ray(p.pos, n_g);
+    w.prev_pos = p.pos;
+    w.prev_n = n_s;
+    return true;
 }
 
 // The MIS prefix sums of the unmodified vertices (see the file comment).
@@ -378,77 +397,71 @@ LT_DEV Vtx camera_vertex(const BdptLaunch& L, uint32_t x, uint32_t y, f3& dir, f
     return cv0;
 }
 
-// Walk kernel body for pixel (x, y): the light subpath (bdpt_commons.glsl:
-// 199-244), the camera subpath (:246-270), the predecessor delta bits and the
-// MIS prefixes; the pixel record carries the subpath lengths, the light
-// vertex 0 fields MIS needs and the RNG counter.
-template <bool COUNT>
-LT_DEV void bdpt_walks(const SceneView& sv, const BdptLaunch& L, const VStore& vs, uint32_t* __restrict__ hdr,
-                       uint32_t x, uint32_t y, const TraceStack& stk, Ctr& c) {
-    RNG seed = {x, y, L.frame_dev ? (*L.frame_dev ^ L.seed) : L.frame_word, 0u};
-    const int D = L.max_depth;
-    int n_light = 0, n_cam = 0;
-    LightV0 l0{0u, 0u, 0.0f};
-#pragma unroll 1
-    for (int sub = 0; sub < 2; sub++) {
-        f3 v0_pos, v0_n, wi, thr;
-        float pdf;
-        bool finite = false;
-        if (sub == 0) {
-            f4 rp;
-            rp.x = rnd(seed);
-            rp.y = rnd(seed);
-            rp.z = rnd(seed);
-            rp.w = rnd(seed);
-            const float rd0 = rnd(seed), rd1 = rnd(seed);
-            float cos_theta, pdf_pos, pdf_dir;
-            uint32_t flags, li;
-            f3 pos, n;
-            const f3 Le = sample_light_Le(sv, rp, rd0, rd1, cos_theta, flags, li, pos, wi, n, pdf_pos, pdf_dir);
-            if (!(pdf_dir > 0.0f)) continue;
-            l0 = {flags, li, pdf_pos};
-            Vtx v0 = zero_vtx();
-            v0.pos = pos;
-            v0.n_s = n;
-            v0.pdf_fwd = pdf_pos;
-            v0.bits = 2u;  // delta 0, side 1
-            v0.thr = Le;
-            vs.store(0, 0, v0);
-            v0_pos = pos;
-            v0_n = n;
-            thr = Le * cos_theta / (pdf_dir * pdf_pos);
-            pdf = pdf_dir;
-            finite = light_finite(flags);
-        } else {
-            const Vtx cv0 = camera_vertex(L, x, y, wi, pdf);
-            vs.store(1, 0, cv0);
-            v0_pos = cv0.pos;
-            v0_n = cv0.n_s;
-            thr = mk3(1.0f);
-        }
-        const int n = random_walk<COUNT>(sv, vs, sub == 1, D, v0_pos, v0_n, wi, finite, thr, pdf, seed, stk, c) + 1;
-        if (sub == 0) {
-            n_light = n;
-            if (!light_finite(l0.flags)) vs.put(0, 1, F_PFWD, l0.pdf_pos * fabsf(dot(wi, vs.get3(0, 1, F_NS))));
-            if (light_delta(l0.flags)) vs.put(0, 0, F_PFWD, 0.0f);
-        } else {
-            n_cam = n;
-        }
+// Vertex 0 and the first direction of subpath w.sub() of the pixel (light:
+// bdpt_commons.glsl:199-244, camera: :246-270).  The light start writes the
+// light vertex 0 fields of the pixel record (zero if the light has no
+// emission sample: pdf_dir <= 0, the pixel has no light subpath, and false
+// is returned).  A delta light's vertex 0 takes forward pdf 0 (:242-243,
+// applied after the walk there; nothing reads it in between).
+LT_DEV bool walk_begin(const SceneView& sv, const BdptLaunch& L, const VStore& vs, uint32_t* __restrict__ hdr,
+                       WalkState& w) {
+    w.set_b(0);
+    if (w.sub() == 0) {
+        f4 rp;
+        rp.x = rnd(w.seed);
+        rp.y = rnd(w.seed);
+        rp.z = rnd(w.seed);
+        rp.w = rnd(w.seed);
+        const float rd0 = rnd(w.seed), rd1 = rnd(w.seed);
+        float cos_theta, pdf_pos, pdf_dir;
+        uint32_t flags, li;
+        f3 pos, n;
+        const f3 Le = sample_light_Le(sv, rp, rd0, rd1, cos_theta, flags, li, pos, w.wi, n, pdf_pos, pdf_dir);
+        const bool ok = pdf_dir > 0.0f;
+        const size_t S = vs.slots;
+        hdr[H_FLAGS * S + vs.slot] = ok ? flags : 0u;
+        hdr[H_IDX * S + vs.slot] = ok ? li : 0u;
+        hdr[H_PDFPOS * S + vs.slot] = __float_as_uint(ok ? pdf_pos : 0.0f);
+        if (!ok) return false;
+        Vtx v0 = zero_vtx();
+        v0.pos = pos;
+        v0.n_s = n;
+        v0.pdf_fwd = light_delta(flags) ? 0.0f : pdf_pos;
+        v0.bits = 2u;  // delta 0, side 1
+        v0.thr = Le;
+        vs.store(0, 0, v0);
+        w.ray_pos = w.prev_pos = pos;
+        w.prev_n = n;
+        w.thr = Le * cos_theta / (pdf_dir * pdf_pos);
+        w.pdf_fwd = pdf_dir;
+        if (light_finite(flags)) w.st |= 2u;
+    } else {
+        const Vtx cv0 = camera_vertex(L, w.seed.x, w.seed.y, w.wi, w.pdf_fwd);
+        vs.store(1, 0, cv0);
+        w.ray_pos = w.prev_pos = cv0.pos;
+        w.prev_n = cv0.n_s;
+        w.thr = mk3(1.0f);
+        w.st &= ~2u;
     }
-    // the predecessor's delta flag as bit 2 of each vertex, then the MIS prefixes
+    return true;
+}
+
+// After both subpaths: the predecessor's delta flag as bit 2 of each vertex,
+// the MIS prefixes, and the rest of the pixel record (subpath lengths, the
+// RNG counter) for the connection kernel.
+LT_DEV void walk_finish(const BdptLaunch& L, const VStore& vs, uint32_t* __restrict__ hdr, const WalkState& w) {
+    const int n_light = w.n_light(), n_cam = w.b() + 1;
+    const size_t S = vs.slots;
+    const bool delta0 = light_delta(hdr[H_FLAGS * S + vs.slot]);
     for (int k = 1; k < n_cam; k++)
         vs.putu(1, k, F_BITS, vs.getu(1, k, F_BITS) | ((vs.getu(1, k - 1, F_BITS) & 1u) << 2));
     for (int k = 0; k < n_light; k++) {
-        const uint32_t dp = k > 0 ? (vs.getu(0, k - 1, F_BITS) & 1u) : (light_delta(l0.flags) ? 1u : 0u);
+        const uint32_t dp = k > 0 ? (vs.getu(0, k - 1, F_BITS) & 1u) : (delta0 ? 1u : 0u);
         vs.putu(0, k, F_BITS, vs.getu(0, k, F_BITS) | (dp << 2));
     }
-    mis_prefixes(vs, n_light, n_cam, light_delta(l0.flags), L.power != 0u);
-    const size_t S = vs.slots;
+    mis_prefixes(vs, n_light, n_cam, delta0, L.power != 0u);
     hdr[H_COUNTS * S + vs.slot] = (uint32_t)n_light | ((uint32_t)n_cam << 8);
-    hdr[H_FLAGS * S + vs.slot] = l0.flags;
-    hdr[H_IDX * S + vs.slot] = l0.idx;
-    hdr[H_PDFPOS * S + vs.slot] = __float_as_uint(l0.pdf_pos);
-    hdr[H_SEEDW * S + vs.slot] = seed.w;
+    hdr[H_SEEDW * S + vs.slot] = w.seed.w;
 }
 
 // Connection kernel body for pixel (x, y): every (s, t) strategy of
@@ -666,39 +679,87 @@ LT_DEV void flush_counts(const Ctr& c, uint32_t lane, unsigned long long* counts
 // Both kernels are persistent: each wave pulls 8x8 pixel tiles of the chunk
 // [tile0, tile0 + n_tiles) from `work`; a pixel's slot in the vertex and
 // record buffers is its position in the chunk.
-template <bool COUNT>
+// The walk kernel: persistent lanes, one bounce per iteration; a lane takes
+// the next pixel of the chunk as soon as its pixel's camera subpath ends
+// (refill when at least regen_min lanes of the wave are idle), so subpaths of
+// uneven length do not leave lanes idle.  A pixel's slot in the vertex and
+// record buffers is its position in the chunk.
 #ifndef LT_BDPT_WALK_WAVES
-#define LT_BDPT_WALK_WAVES 1
+#define LT_BDPT_WALK_WAVES 4
 #endif
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_BDPT_WALK_WAVES))) void k_bdpt_walk(SceneView sv, BdptLaunch L, float* __restrict__ verts,
-                                                           uint32_t* __restrict__ hdr, uint32_t slots, uint32_t tile0,
-                                                           uint32_t n_tiles, uint32_t* __restrict__ work,
-                                                           uint32_t* __restrict__ spill,
-                                                           unsigned long long* __restrict__ counts) {
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_BDPT_WALK_WAVES))) void k_bdpt_walk(
+    SceneView sv, BdptLaunch L, float* __restrict__ verts, uint32_t* __restrict__ hdr, uint32_t slots, uint32_t tile0,
+    uint32_t n_tiles, uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+    unsigned long long* __restrict__ counts) {
     __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t tiles_x = (L.width + 7) / 8;
+    const uint32_t n_items = n_tiles * 64u;
+    const uint32_t frame_word = L.frame_dev ? (*L.frame_dev ^ L.seed) : L.frame_word;
+    const int D = L.max_depth;
+    const uint32_t nv = (uint32_t)D + 1u;
     Ctr c{0, 0, 0, 0};
+    WalkState w;
+    bool has = false, exhausted = false;  // exhausted is wave-uniform
     for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(work, 1u);
-        t = __shfl(t, 0);
-        if (t >= n_tiles) break;
-        const uint32_t g = tile0 + t;
-        const uint32_t x = (g % tiles_x) * 8 + (lane & 7), y = (g / tiles_x) * 8 + (lane >> 3);
-        if (x >= L.width || y >= L.height) continue;
-        const VStore vs{verts, slots, (uint32_t)L.max_depth + 1u, t * 64u + lane};
-        bdpt_walks<COUNT>(sv, L, vs, hdr, x, y, stk, c);
+        if (!exhausted) {
+            const unsigned long long idle = __ballot(!has);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= L.regen_min || n_idle == 64u) {
+                const int leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(work, n_idle);
+                base = __shfl(base, leader);
+                if (base + n_idle >= n_items) exhausted = true;
+                if (!has) {
+                    const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (i < n_items) {
+                        const uint32_t g = tile0 + (i >> 6), q = i & 63u;
+                        const uint32_t x = (g % tiles_x) * 8 + (q & 7u), y = (g / tiles_x) * 8 + (q >> 3);
+                        if (x < L.width && y < L.height) {
+                            w.seed = {x, y, frame_word, 0u};
+                            w.slot = i;
+                            w.st = 0u;
+                            const VStore vs{verts, slots, nv, i};
+                            if (!walk_begin(sv, L, vs, hdr, w)) {
+                                w.st = 1u;  // no light subpath: n_light 0
+                                walk_begin(sv, L, vs, hdr, w);
+                            }
+                            has = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has) {
+            const VStore vs{verts, slots, nv, w.slot};
+            float pdf_pos_inf = 0.0f;
+            if (w.st == 0u) pdf_pos_inf = __uint_as_float(hdr[H_PDFPOS * (size_t)slots + w.slot]);
+            if (D == 0 || !walk_step<COUNT>(sv, vs, D, w, pdf_pos_inf, stk, c)) {
+                if (w.sub() == 0) {  // the light subpath ended: the camera subpath starts
+                    w.st = 1u | ((uint32_t)(w.b() + 1) << 16);
+                    walk_begin(sv, L, vs, hdr, w);
+                } else {
+                    walk_finish(L, vs, hdr, w);
+                    has = false;
+                }
+            }
+        }
     }
     if (COUNT) flush_counts(c, lane, counts);
 }
 
-template <bool COUNT>
 #ifndef LT_BDPT_CONN_WAVES
 #define LT_BDPT_CONN_WAVES 1
 #endif
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_BDPT_CONN_WAVES))) void k_bdpt_connect(SceneView sv, BdptLaunch L, float* __restrict__ verts,
                                                               const uint32_t* __restrict__ hdr, uint32_t slots,
                                                               uint32_t tile0, uint32_t n_tiles,
@@ -760,9 +821,34 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 
 }  // namespace
 
+// Grids of the walk and connection kernels: the workgroups every CU holds at
+// once (persistent kernels), unless the tuning fixes a count.
+static void bdpt_grids(lt_context* ctx, int count, uint32_t& walk, uint32_t& conn) {
+    const uint32_t fixed = ctx->tuning.bdpt_blocks_per_cu;
+    const void* kw = count ? reinterpret_cast<const void*>(&k_bdpt_walk<true>)
+                           : reinterpret_cast<const void*>(&k_bdpt_walk<false>);
+    const void* kc = count ? reinterpret_cast<const void*>(&k_bdpt_connect<true>)
+                           : reinterpret_cast<const void*>(&k_bdpt_connect<false>);
+    walk = (uint32_t)ctx->num_cus * resident_blocks(kw, fixed);
+    conn = (uint32_t)ctx->num_cus * resident_blocks(kc, fixed);
+}
+
+size_t bdpt_spill_threads(lt_context* ctx) {
+    size_t m = 0;
+    for (int count = 0; count < 2; count++) {
+        uint32_t w, c;
+        bdpt_grids(ctx, count, w, c);
+        m = std::max(m, (size_t)std::max(w, c));
+    }
+    return m * kTraceBlock;
+}
+
 lt_status bdpt_alloc(lt_context* ctx, BdptBuffers& b, uint32_t width, uint32_t height, int32_t max_depth) {
     const uint32_t tiles = ((width + 7) / 8) * ((height + 7) / 8);
-    b.chunk_tiles = tiles < kBdptChunkTiles ? tiles : kBdptChunkTiles;
+    const size_t tile_bytes = (size_t)64 * 2 * (max_depth + 1) * kVtxFields * sizeof(float);
+    const size_t fit = kBdptChunkBytes / tile_bytes;
+    const size_t want = ctx->tuning.bdpt_chunk_tiles ? ctx->tuning.bdpt_chunk_tiles : (fit > 64 ? fit : 64);
+    b.chunk_tiles = (uint32_t)(tiles < want ? tiles : want);
     b.slots = b.chunk_tiles * 64u;
     b.max_depth = max_depth;
     b.pixels = (size_t)width * height;
@@ -791,30 +877,32 @@ lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L0, uin
     if (L0.max_depth != b.max_depth || (size_t)L0.width * L0.height != b.pixels)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "BDPT launch does not match its buffers");
     const uint32_t tiles = ((L0.width + 7) / 8) * ((L0.height + 7) / 8);
-    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.bdpt_blocks_per_cu;
-    lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
+    uint32_t walk_blocks, conn_blocks;
+    bdpt_grids(ctx, count, walk_blocks, conn_blocks);
+    lt_status st = ensure_spill(ctx, bdpt_spill_threads(ctx));
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
     if (count) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, (size_t)8, 0u);
     const uint32_t n = (uint32_t)b.pixels;
-    const dim3 grid((unsigned)blocks), block(kTraceBlock);
+    const dim3 wgrid(walk_blocks), cgrid(conn_blocks), block(kTraceBlock);
     for (uint32_t f = 0; f < frames; f++) {
         BdptLaunch L = L0;
         L.frame_word = (frame_num + f) ^ seed;
         L.seed = seed;
+        L.regen_min = ctx->tuning.regen_min ? ctx->tuning.regen_min : 4u;
         for (uint32_t t0 = 0; t0 < tiles; t0 += b.chunk_tiles) {
             const uint32_t nt = tiles - t0 < b.chunk_tiles ? tiles - t0 : b.chunk_tiles;
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, (size_t)2, 0u);
             if (count) {
-                hipLaunchKernelGGL((k_bdpt_walk<true>), grid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
+                hipLaunchKernelGGL((k_bdpt_walk<true>), wgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
                                    t0, nt, ctx->counters, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_bdpt_connect<true>), grid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
+                hipLaunchKernelGGL((k_bdpt_connect<true>), cgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
                                    b.slots, t0, nt, b.fcol, b.splat, ctx->counters + 1, ctx->spill, cnt);
             } else {
-                hipLaunchKernelGGL((k_bdpt_walk<false>), grid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
+                hipLaunchKernelGGL((k_bdpt_walk<false>), wgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
                                    t0, nt, ctx->counters, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_bdpt_connect<false>), grid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
+                hipLaunchKernelGGL((k_bdpt_connect<false>), cgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
                                    b.slots, t0, nt, b.fcol, b.splat, ctx->counters + 1, ctx->spill, cnt);
             }
         }

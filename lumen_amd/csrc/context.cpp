@@ -59,6 +59,13 @@ lt_status ensure_spill(lt_context* ctx, size_t threads) {
     return LT_OK;
 }
 
+uint32_t resident_blocks(const void* kernel, uint32_t fixed) {
+    if (fixed) return fixed;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kTraceBlock, 0) != hipSuccess || n < 1) return 1u;
+    return (uint32_t)n;
+}
+
 static void free_scene(DeviceScene& s) {
     void* ptrs[] = {s.vtx, s.idx, s.prims, s.mats, s.lights, s.cones, s.texels, s.texs, s.rough};
     for (void* p : ptrs)
@@ -308,6 +315,7 @@ lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
     if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
     if (t->refill_min) ctx->tuning.refill_min = t->refill_min;
     if (t->regen_min) ctx->tuning.regen_min = t->regen_min;
+    if (t->bdpt_chunk_tiles) ctx->tuning.bdpt_chunk_tiles = t->bdpt_chunk_tiles;
     ctx->generation++;
     return LT_OK;
 }

@@ -87,7 +87,7 @@ struct lt_context {
     size_t hit_buf_bytes = 0;
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
     float last_ms = 0.0f;
-    lt_tuning tuning{10u, 8u, 32u, 4u, 4u, {}};
+    lt_tuning tuning{10u, 8u, 32u, 0u, 4u, {}};
     // bumped whenever device state that launches capture by pointer or value
     // changes (scene upload, BVH build, spill reallocation, launch shapes):
     // a captured frame graph older than this is stale
@@ -107,6 +107,9 @@ SceneView make_scene_view(const lt_context* ctx);
 // persistent grid: number of resident threads the spill buffer is sized for
 lt_status ensure_spill(lt_context* ctx, size_t threads);
 constexpr int kTraceBlock = 128;
+// workgroups of kTraceBlock threads of `kernel` that one CU holds at once
+// (its register and LDS use), at least 1; `fixed` (a tuning value) wins if set
+uint32_t resident_blocks(const void* kernel, uint32_t fixed);
 
 // builder / kernels (implemented in the .hip files)
 lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* stats);

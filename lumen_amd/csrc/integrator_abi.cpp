@@ -209,8 +209,7 @@ static lt_status capture_frame(lt_integrator* it, const lt_frame_params* p) {
     drop_graph(it);
     if (!it->d_frame) LT_HIP(ctx, hipMalloc(&it->d_frame, 256));
     const size_t pblocks = (size_t)ctx->num_cus * ctx->tuning.path_blocks_per_cu;
-    const size_t bblocks = (size_t)ctx->num_cus * ctx->tuning.bdpt_blocks_per_cu;
-    lt_status st = ensure_spill(ctx, (it->kind == LT_INTEGRATOR_BDPT ? bblocks : pblocks) * kTraceBlock);
+    lt_status st = ensure_spill(ctx, it->kind == LT_INTEGRATOR_BDPT ? bdpt_spill_threads(ctx) : pblocks * kTraceBlock);
     if (st == LT_OK && it->kind == LT_INTEGRATOR_PATH && it->path.strategy == LT_PATH_WAVEFRONT)
         st = wave_alloc(ctx, it->wave, it->width * it->height);
     if (st != LT_OK) return st;

@@ -19,6 +19,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "dev_scene.h"
 #include "dev_trace.h"
 #include "vcm.h"
@@ -700,14 +702,22 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
                      uint64_t* d_counts) {
     if (L0.max_depth != b.max_depth || L0.n_paths != b.n_paths)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "VCM launch does not match its buffers");
-    const size_t blocks = (size_t)ctx->num_cus * ctx->tuning.bdpt_blocks_per_cu;
-    lt_status st = ensure_spill(ctx, blocks * kTraceBlock);
+    // persistent grids: the workgroups every CU holds at once, unless the tuning fixes a count
+    const uint32_t fixed = ctx->tuning.bdpt_blocks_per_cu;
+    const uint32_t light_blocks =
+        ctx->num_cus * resident_blocks(count ? reinterpret_cast<const void*>(&k_vcm_light<true>)
+                                             : reinterpret_cast<const void*>(&k_vcm_light<false>),
+                                       fixed);
+    const uint32_t eye_blocks = ctx->num_cus * resident_blocks(count ? reinterpret_cast<const void*>(&k_vcm_eye<true>)
+                                                                     : reinterpret_cast<const void*>(&k_vcm_eye<false>),
+                                                               fixed);
+    lt_status st = ensure_spill(ctx, (size_t)std::max(light_blocks, eye_blocks) * kTraceBlock);
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
     if (count) fill(ctx, (uint32_t*)d_counts, 8);
     const size_t N = b.n_paths, cells = 10 * N;
-    const dim3 grid((unsigned)blocks), block(kTraceBlock);
+    const dim3 lgrid(light_blocks), egrid(eye_blocks), block(kTraceBlock);
     const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.phot};
     for (uint32_t f = 0; f < frames; f++) {
         VcmLaunch L = L0;
@@ -724,10 +734,10 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
             fill(ctx, b.count, cells + 1);
         }
         if (count)
-            hipLaunchKernelGGL((k_vcm_light<true>), grid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
+            hipLaunchKernelGGL((k_vcm_light<true>), lgrid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
                                b.key, b.count, ctx->counters, ctx->spill, cnt);
         else
-            hipLaunchKernelGGL((k_vcm_light<false>), grid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
+            hipLaunchKernelGGL((k_vcm_light<false>), lgrid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
                                b.key, b.count, ctx->counters, ctx->spill, cnt);
         if (L.use_vm && L.corrected) {
             LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, b.scan_bytes, b.count, b.start,
@@ -737,10 +747,10 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
                                b.verts, b.cnt, b.start, b.fill, b.phot);
         }
         if (count)
-            hipLaunchKernelGGL((k_vcm_eye<true>), grid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
+            hipLaunchKernelGGL((k_vcm_eye<true>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
                                ctx->counters + 1, ctx->spill, cnt);
         else
-            hipLaunchKernelGGL((k_vcm_eye<false>), grid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
+            hipLaunchKernelGGL((k_vcm_eye<false>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
                                ctx->counters + 1, ctx->spill, cnt);
     }
     LT_HIP(ctx, hipGetLastError());

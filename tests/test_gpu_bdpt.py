@@ -102,3 +102,27 @@ def test_bdpt_frames_accumulate(gpu_ctx):
     seq = a.output_tex
     a.destroy()
     assert np.allclose(one, seq, rtol=1e-4, atol=1e-5)
+
+
+def test_bdpt_chunked_passes_equal_one_pass(gpu_ctx):
+    """A frame split into several walk/connection passes (bdpt_chunk_tiles,
+    ragged last pass) equals the one-pass frame: a pixel's subpaths depend
+    only on its RNG stream, so only the order of the splat atomics differs."""
+    print("[gpu] bdpt chunked passes", flush=True)
+    sc = la.Scene.procedural("caustics", seed=1)
+    W, H = 48, 40  # 30 tiles: passes of 7, 7, 7, 7, 2
+    imgs = []
+    ctx2 = la.Context(0)
+    ctx2.set_tuning(bdpt_chunk_tiles=7)
+    try:
+        for ctx in (gpu_ctx, ctx2):
+            it = la.BDPT(sc, ctx, W, H, path_length=8)
+            it.seed = 3
+            it.init()
+            it.create_accel()
+            it.render(spp=2)
+            imgs.append(it.output_tex)
+            it.destroy()
+    finally:
+        ctx2.close()
+    assert np.allclose(imgs[0], imgs[1], rtol=1e-5, atol=1e-6)
