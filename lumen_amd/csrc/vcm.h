@@ -24,6 +24,17 @@ struct VcmLaunch {
     uint32_t corrected;  // 0: the reference estimator, 1: corrected VCM
 };
 
+// A merge query of the corrected estimator, 80 B: an eye vertex and the
+// camera subpath state merging needs, written by the eye pass and gathered
+// by the merge pass.
+struct VcmQuery {
+    float4 a;  // pos.xyz, d_vcm
+    float4 b;  // n_s.xyz, d_vm
+    float4 c;  // wo.xyz, uv.x
+    float4 d;  // throughput.xyz, uv.y
+    uint4 e;   // material_idx, pixel, side | depth << 1, 0
+};
+
 // One stored light vertex (VCMVertex, vcm_commons.h:88-103), 96 B.
 struct VcmVertex {
     float4 a;  // pos.xyz, d_vcm
@@ -52,7 +63,7 @@ struct VcmPhoton {
 //    the photons themselves, bucket by bucket, so a gather streams each
 //    bucket's records contiguously.
 //  - splat: light-tracing splats (float3 per pixel), read and cleared by the
-//    eye pass.
+//    eye pass (by the resolve pass when merges are gathered separately).
 struct VcmBuffers {
     VcmVertex* verts = nullptr;
     uint32_t* cnt = nullptr;
@@ -64,6 +75,11 @@ struct VcmBuffers {
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     float* splat = nullptr;
+    // corrected estimator with merging: the eye pass's colours (merges added
+    // by the merge pass) and its merge queries (at most max_depth per pixel)
+    float4* fcol = nullptr;
+    VcmQuery* queries = nullptr;
+    uint32_t* qctr = nullptr;  // queries written
     uint32_t n_paths = 0;
     int32_t max_depth = 0;
 };

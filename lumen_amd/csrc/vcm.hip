@@ -478,93 +478,285 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
     return res;
 }
 
-// vcm_eye.rgen:42-79 + vcm_commons.glsl:477-631 for pixel (x, y)
-template <bool COUNT>
-LT_DEV f3 eye_path(const SceneView& sv, const VcmLaunch& L, const Grid& G, const Camera& cam, uint32_t x, uint32_t y,
-                   const TraceStack& stk, Ctr& c) {
-    RNG seed = {x, y, L.eye_word, 0u};
+// ---- merge pass (corrected estimator) ----
+// vcm_commons.glsl:251-296 summed over every photon in range, for the merge
+// queries the eye pass wrote.  A whole wave serves one query at a time: the
+// query and its material are wave-uniform (scalar loads and registers, and
+// the BSDF's branches on the material are uniform), the lanes load the
+// photon ranges of up to 64 of its cells at once, a wave scan lays the
+// cells' photons end to end and each lane takes every 64th photon.  A query
+// over a dense caustic costs ceil(photons / 64) coalesced steps and no lane
+// waits on another lane's serial loop.  The wave's sum is added to the
+// pixel's colour with float atomics (the order of the sum is the only
+// difference from a serial loop).
+#ifndef LT_VCM_MERGE_WAVES
+#define LT_VCM_MERGE_WAVES 3
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_MERGE_WAVES))) void k_vcm_merge(
+    SceneView sv, VcmLaunch L, Grid Gr, const VcmQuery* __restrict__ queries, const uint32_t* __restrict__ n_queries,
+    float4* __restrict__ fcol) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t Q = *n_queries;
+    const float r2 = L.radius * L.radius;
+    const float norm = 1.0f / (PI * r2 * (float)L.n_paths);
+    // queries w, w + waves, ...: neighbouring queries (neighbouring pixels)
+    // cost alike, so the interleave balances
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64u);
+    for (uint32_t qi = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / 64u));
+         qi < Q; qi += nwaves) {
+        const VcmQuery& qr = queries[qi];
+        const float4 qa = qr.a, qb = qr.b, qc = qr.c, qd = qr.d;
+        const uint4 qe = qr.e;
+        const f3 qpos = {qa.x, qa.y, qa.z}, qn = {qb.x, qb.y, qb.z}, qwo = {qc.x, qc.y, qc.z};
+        VState cs;
+        cs.thr = {qd.x, qd.y, qd.z};
+        cs.d_vcm = qa.w;
+        cs.d_vm = qb.w;
+        const bool side = (qe.z & 1u) != 0u;
+        const int depth = (int)(qe.z >> 1);
+        const Mat mat = load_material(sv, qe.x, f2{qc.w, qd.w});
+        int lo[3], hi[3];
+        grid_idx(L, qpos - mk3(L.radius), lo);
+        grid_idx(L, qpos + mk3(L.radius), hi);
+        const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+        const int ncells = (hi[0] - lo[0] + 1) * ny * nz;
+        f3 part = mk3(0.0f);
+        for (int cbase = 0; cbase < ncells; cbase += 64) {
+            const int ci = cbase + (int)lane;  // this lane's cell of the chunk: its photon range
+            uint32_t st = 0u, cnt = 0u;
+            if (ci < ncells) {
+                const int cx = lo[0] + ci / (ny * nz), cy = lo[1] + (ci / nz) % ny, cz = lo[2] + ci % nz;
+                const uint32_t h = cell_hash(cx, cy, cz, L.n_paths);
+                st = Gr.start[h];
+                cnt = Gr.start[h + 1] - st;
+            }
+            uint32_t incl = cnt;  // inclusive scan over the wave
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, (unsigned)o);
+                if ((int)lane >= o) incl += v;
+            }
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            for (uint32_t t0 = 0; t0 < total; t0 += 64u) {
+                const uint32_t t = t0 + lane;
+                int j = 0;  // the cell holding photon t: the first lane whose scan exceeds t
+                for (int step = 32; step > 0; step >>= 1) {
+                    const uint32_t v = __shfl(incl, j + step - 1);
+                    if (v <= t) j += step;
+                }
+                j = j < 63 ? j : 63;
+                const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
+                if (t >= total) continue;
+                const VcmPhoton& ph = Gr.phot[sj + (t - ej)];
+                const float4 a = ph.a, b = ph.b, c = ph.c, d = ph.d;  // one 64-B record, loaded at once
+                const f3 ppos = {a.x, a.y, a.z};
+                const f3 pp = qpos - ppos;
+                const float dist_sqr = dot(pp, pp);
+                if (dist_sqr > r2) continue;
+                const int cj = cbase + j;
+                int g[3];
+                grid_idx(L, ppos, g);
+                if (g[0] != lo[0] + cj / (ny * nz) || g[1] != lo[1] + (cj / nz) % ny || g[2] != lo[2] + cj % nz)
+                    continue;  // another cell of the bucket
+                f3 r;
+                if (merge_one(L, dist_sqr, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, a.w, b.w,
+                              __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
+                    part += r;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            part.x += __shfl_xor(part.x, o);
+            part.y += __shfl_xor(part.y, o);
+            part.z += __shfl_xor(part.z, o);
+        }
+        if (lane == 0u && !is_zero(part)) {
+            float* fc = reinterpret_cast<float*>(fcol + qe.y);
+            atomicAdd(fc + 0, part.x);
+            atomicAdd(fc + 1, part.y);
+            atomicAdd(fc + 2, part.z);
+        }
+    }
+}
+
+// The eye pass's colour plus the merges and the splats into the running
+// mean (vcm_eye.rgen:80-97); a NaN colour drops the sample and leaves the
+// pixel's splat for the next frame.
+__global__ void k_vcm_resolve(const float4* __restrict__ fcol, float* __restrict__ splat, float4* __restrict__ accum,
+                              uint32_t n, uint32_t frame_num) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 fc = fcol[i];
+    f3 col = mk3(fc.x, fc.y, fc.z);
+    if (isnan(luminance(col))) return;
+    float* sp = splat + 3 * (size_t)i;
+    col += mk3(sp[0], sp[1], sp[2]);
+    sp[0] = sp[1] = sp[2] = 0.0f;
+    float4 a = accum[i];
+    if (frame_num > 0) {
+        const float w = 1.0f / (float)(frame_num + 1);
+        a.x = a.x * (1.0f - w) + col.x * w;
+        a.y = a.y * (1.0f - w) + col.y * w;
+        a.z = a.z * (1.0f - w) + col.z * w;
+    } else {
+        a.x = col.x;
+        a.y = col.y;
+        a.z = col.z;
+    }
+    a.w = 1.0f;
+    accum[i] = a;
+}
+
+// The camera subpath of one pixel (vcm_eye.rgen:42-79 + vcm_commons.glsl:
+// 477-631), one vertex per step of the wave's loop so that the whole wave
+// reaches the merge of every depth together.
+struct EyeState {
+    RNG seed;
+    VState cs;
+    f3 col;
+    uint32_t lp;
+    bool alive;
+};
+
+LT_DEV void eye_start(const VcmLaunch& L, const Camera& cam, uint32_t x, uint32_t y, EyeState& e) {
+    e.seed = {x, y, L.eye_word, 0u};
     const float screen_size = (float)L.n_paths;
-    const float r0 = rnd(seed), r1 = rnd(seed);
+    const float r0 = rnd(e.seed), r1 = rnd(e.seed);
     const float ux = ((float)x + 0.5f + (r0 - 0.5f)) / (float)L.width;
     const float uy = ((float)y + 0.5f + (r1 - 0.5f)) / (float)L.height;
     const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
     const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
     const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
     const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
-    VState cs;
-    cs.wi = {d4.x, d4.y, d4.z};
-    cs.pos = cam.pos;
-    cs.thr = mk3(1.0f);
-    cs.n_s = cam.nrm;
-    const float ct = fabsf(dot(cs.n_s, cs.wi));
-    cs.d_vcm = cam.area * screen_size * ct * ct * ct;
-    cs.d_vc = 0.0f;
-    cs.d_vm = 0.0f;
-    uint32_t lp = (uint32_t)(rnd(seed) * screen_size);
-    if (lp >= L.n_paths) lp = L.n_paths - 1u;  // float rounding of rand * N at very large N
-    f3 col = mk3(0.0f);
-    const float norm = 1.0f / (PI * (L.radius * L.radius) * screen_size);
-    for (int depth = 1;; depth++) {
-        Payload pl;
-        if (!closest<COUNT>(sv, cs.pos, cs.wi, pl, stk, c)) {
-            col += cs.thr * ld3(L.sky_col);
-            break;
-        }
-        f3 wo = cs.pos - pl.pos;
-        const float dist = length(pl.pos - cs.pos);
-        const float dist_sqr = dist * dist;
-        wo /= dist;
-        f3 n_s = pl.n_s, n_g = pl.n_g;
-        bool side = true;
-        if (dot(pl.n_g, wo) < 0.0f) n_g = -n_g;
-        if (dot(n_g, n_s) < 0.0f) {
-            n_s = -n_s;
-            side = false;
-        }
-        const float cos_wo = fabsf(dot(wo, n_s));
-        const Mat mat = load_material(sv, pl.material_idx, pl.uv);
-        const bool spec = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
-        cs.d_vcm *= dist_sqr;
-        cs.d_vcm /= cos_wo;
-        cs.d_vc /= cos_wo;
-        cs.d_vm /= cos_wo;
-        if (luminance(mat.emissive) > 0.0f) col += cs.thr * light_radiance(sv, L, mat, cs, pl, depth);
-        float pdf_rev = 0.0f;
-        if (!spec && depth < L.max_depth) col += connect_light<COUNT>(sv, L, n_s, wo, mat, side, cs, pl, seed, pdf_rev, stk, c);
-        if (!spec) col += connect_light_vertices<COUNT>(sv, L, G, lp, depth, n_s, wo, mat, side, cs, pdf_rev, pl, stk, c);
-        if (!spec && L.use_vm) col += merge(L, G, depth, n_s, wo, mat, side, cs, pl, norm);
-        if (depth >= L.max_depth) break;
-        float pdf_d, cos_t;
-        const float s0 = rnd(seed), s1 = rnd(seed), s2 = rnd(seed);
-        const f3 f = sample_bsdf(n_s, wo, mat, 1u, side, cs.wi, pdf_d, cos_t, s0, s1, s2);
-        const bool transm = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
-        const bool same_hem = dot(cs.wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
-        if (is_zero(f) || pdf_d == 0.0f || (!same_hem && !transm)) break;
-        pdf_rev = pdf_d;
-        if (!spec) pdf_rev = bsdf_pdf(mat, n_s, cs.wi, wo, side);
-        const float ac = fabsf(cos_t);
-        cs.pos = offset_ray(pl.pos, n_g);
-        if (!spec) {
-            cs.d_vc = (ac / pdf_d) * (L.eta_vm + cs.d_vcm + pdf_rev * cs.d_vc);
-            cs.d_vm = (ac / pdf_d) * (1.0f + cs.d_vcm * L.eta_vc + pdf_rev * cs.d_vm);
-            cs.d_vcm = 1.0f / pdf_d;
-        } else {
-            cs.d_vcm = 0.0f;
-            cs.d_vc *= ac;
-            cs.d_vm *= ac;
-        }
-        cs.thr *= f * ac / pdf_d;
-        cs.n_s = n_s;
-    }
-    return col;
+    e.cs.wi = {d4.x, d4.y, d4.z};
+    e.cs.pos = cam.pos;
+    e.cs.thr = mk3(1.0f);
+    e.cs.n_s = cam.nrm;
+    const float ct = fabsf(dot(e.cs.n_s, e.cs.wi));
+    e.cs.d_vcm = cam.area * screen_size * ct * ct * ct;
+    e.cs.d_vc = 0.0f;
+    e.cs.d_vm = 0.0f;
+    e.lp = (uint32_t)(rnd(e.seed) * screen_size);
+    if (e.lp >= L.n_paths) e.lp = L.n_paths - 1u;  // float rounding of rand * N at very large N
+    e.col = mk3(0.0f);
+    e.alive = true;
 }
 
+// With the corrected estimator and merging, the merges are left to the merge
+// pass: each non-specular vertex appends a query (wave-aggregated counter).
 template <bool COUNT>
+LT_DEV f3 eye_wave(const SceneView& sv, const VcmLaunch& L, const Grid& G, const Camera& cam, uint32_t x, uint32_t y,
+                   bool valid, VcmQuery* __restrict__ queries, uint32_t* __restrict__ n_queries,
+                   const TraceStack& stk, Ctr& c) {
+    EyeState e;
+    e.alive = false;
+    e.col = mk3(0.0f);
+    if (valid) eye_start(L, cam, x, y, e);
+    const float norm = 1.0f / (PI * (L.radius * L.radius) * (float)L.n_paths);
+    const bool coop = L.use_vm && L.corrected;
+    for (int depth = 1; __ballot(e.alive) != 0ull; depth++) {
+        bool want = false, side = true, spec = false;
+        f3 pos = mk3(0.0f), n_s = mk3(0.0f), n_g = mk3(0.0f), wo = mk3(0.0f);
+        uint32_t mat_idx = 0u;
+        f2 uv = {0.0f, 0.0f};
+        if (e.alive) {
+            Payload pl;
+            if (!closest<COUNT>(sv, e.cs.pos, e.cs.wi, pl, stk, c)) {
+                e.col += e.cs.thr * ld3(L.sky_col);
+                e.alive = false;
+            } else {
+                pos = pl.pos;
+                mat_idx = pl.material_idx;
+                uv = pl.uv;
+                wo = e.cs.pos - pl.pos;
+                const float dist = length(pl.pos - e.cs.pos);
+                const float dist_sqr = dist * dist;
+                wo /= dist;
+                n_s = pl.n_s;
+                n_g = pl.n_g;
+                if (dot(pl.n_g, wo) < 0.0f) n_g = -n_g;
+                if (dot(n_g, n_s) < 0.0f) {
+                    n_s = -n_s;
+                    side = false;
+                }
+                const float cos_wo = fabsf(dot(wo, n_s));
+                const Mat mat = load_material(sv, pl.material_idx, pl.uv);
+                spec = (mat.bsdf_props & LT_BSDF_FLAG_SPECULAR) == LT_BSDF_FLAG_SPECULAR;
+                e.cs.d_vcm *= dist_sqr;
+                e.cs.d_vcm /= cos_wo;
+                e.cs.d_vc /= cos_wo;
+                e.cs.d_vm /= cos_wo;
+                if (luminance(mat.emissive) > 0.0f) e.col += e.cs.thr * light_radiance(sv, L, mat, e.cs, pl, depth);
+                float pdf_rev = 0.0f;
+                if (!spec && depth < L.max_depth)
+                    e.col += connect_light<COUNT>(sv, L, n_s, wo, mat, side, e.cs, pl, e.seed, pdf_rev, stk, c);
+                if (!spec)
+                    e.col += connect_light_vertices<COUNT>(sv, L, G, e.lp, depth, n_s, wo, mat, side, e.cs, pdf_rev, pl,
+                                                           stk, c);
+                if (!spec && L.use_vm && !L.corrected)
+                    e.col += merge(L, G, depth, n_s, wo, mat, side, e.cs, pl, norm);
+                want = !spec && coop;
+            }
+        }
+        if (coop) {
+            const unsigned long long qm = __ballot(want);
+            if (qm) {
+                const uint32_t lane = threadIdx.x & 63u;
+                const int leader = __ffsll((long long)qm) - 1;
+                uint32_t qbase = 0;
+                if ((int)lane == leader) qbase = atomicAdd(n_queries, (uint32_t)__popcll(qm));
+                qbase = __shfl(qbase, leader);
+                if (want) {
+                    VcmQuery& q = queries[qbase + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull))];
+                    q.a = make_float4(pos.x, pos.y, pos.z, e.cs.d_vcm);
+                    q.b = make_float4(n_s.x, n_s.y, n_s.z, e.cs.d_vm);
+                    q.c = make_float4(wo.x, wo.y, wo.z, uv.x);
+                    q.d = make_float4(e.cs.thr.x, e.cs.thr.y, e.cs.thr.z, uv.y);
+                    q.e = make_uint4(mat_idx, y * L.width + x, (side ? 1u : 0u) | ((uint32_t)depth << 1), 0u);
+                }
+            }
+        }
+        if (e.alive) {
+            if (depth >= L.max_depth) {
+                e.alive = false;
+            } else {
+                const Mat mat = load_material(sv, mat_idx, uv);
+                float pdf_d, cos_t;
+                const float s0 = rnd(e.seed), s1 = rnd(e.seed), s2 = rnd(e.seed);
+                const f3 f = sample_bsdf(n_s, wo, mat, 1u, side, e.cs.wi, pdf_d, cos_t, s0, s1, s2);
+                const bool transm = (mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) == LT_BSDF_FLAG_TRANSMISSION;
+                const bool same_hem = dot(e.cs.wi, n_s) > 0.0f && dot(wo, n_s) > 0.0f;
+                if (is_zero(f) || pdf_d == 0.0f || (!same_hem && !transm)) {
+                    e.alive = false;
+                } else {
+                    float pdf_rev = pdf_d;
+                    if (!spec) pdf_rev = bsdf_pdf(mat, n_s, e.cs.wi, wo, side);
+                    const float ac = fabsf(cos_t);
+                    e.cs.pos = offset_ray(pos, n_g);
+                    if (!spec) {
+                        e.cs.d_vc = (ac / pdf_d) * (L.eta_vm + e.cs.d_vcm + pdf_rev * e.cs.d_vc);
+                        e.cs.d_vm = (ac / pdf_d) * (1.0f + e.cs.d_vcm * L.eta_vc + pdf_rev * e.cs.d_vm);
+                        e.cs.d_vcm = 1.0f / pdf_d;
+                    } else {
+                        e.cs.d_vcm = 0.0f;
+                        e.cs.d_vc *= ac;
+                        e.cs.d_vm *= ac;
+                    }
+                    e.cs.thr *= f * ac / pdf_d;
+                    e.cs.n_s = n_s;
+                }
+            }
+        }
+    }
+    return e.col;
+}
+
 #ifndef LT_VCM_EYE_WAVES
 #define LT_VCM_EYE_WAVES 2
 #endif
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_EYE_WAVES, LT_VCM_EYE_WAVES))) void k_vcm_eye(SceneView sv, VcmLaunch L, Grid G,
                                                          float* __restrict__ splat, float4* __restrict__ accum,
+                                                         float4* __restrict__ fcol, VcmQuery* __restrict__ queries,
+                                                         uint32_t* __restrict__ n_queries,
                                                          uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
                                                          unsigned long long* __restrict__ counts) {
     __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
@@ -581,10 +773,15 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         t = __shfl(t, 0);
         if (t >= n_tiles) break;
         const uint32_t x = (t % tiles_x) * 8 + (lane & 7), y = (t / tiles_x) * 8 + (lane >> 3);
-        if (x >= L.width || y >= L.height) continue;
-        f3 col = eye_path<COUNT>(sv, L, G, cam, x, y, stk, c);
-        if (isnan(luminance(col))) continue;  // vcm_eye.rgen:80-83: the splat stays for the next frame
+        const bool valid = x < L.width && y < L.height;
+        f3 col = eye_wave<COUNT>(sv, L, G, cam, x, y, valid, queries, n_queries, stk, c);
+        if (!valid) continue;
         const size_t i = (size_t)y * L.width + x;
+        if (fcol) {  // merges and the resolve follow in their own passes
+            fcol[i] = make_float4(col.x, col.y, col.z, 0.0f);
+            continue;
+        }
+        if (isnan(luminance(col))) continue;  // vcm_eye.rgen:80-83: the splat stays for the next frame
         float* sp = splat + 3 * i;
         col += mk3(sp[0], sp[1], sp[2]);
         sp[0] = sp[1] = sp[2] = 0.0f;
@@ -691,7 +888,8 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
 }
 
 void vcm_free(VcmBuffers& b) {
-    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.phot, b.scan_tmp, b.splat};
+    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.phot, b.scan_tmp, b.splat, b.fcol, b.queries,
+                    b.qctr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = VcmBuffers{};
@@ -713,6 +911,14 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
                                                                fixed);
     lt_status st = ensure_spill(ctx, (size_t)std::max(light_blocks, eye_blocks) * kTraceBlock);
     if (st != LT_OK) return st;
+    // corrected estimator with merging: a merge pass over the eye pass's queries
+    const bool merge_pass = L0.use_vm && L0.corrected;
+    if (merge_pass && !b.queries) {
+        LT_HIP(ctx, hipMalloc(&b.fcol, (size_t)b.n_paths * sizeof(float4)));
+        LT_HIP(ctx, hipMalloc(&b.queries, (size_t)b.n_paths * (size_t)b.max_depth * sizeof(VcmQuery)));
+        LT_HIP(ctx, hipMalloc(&b.qctr, 2 * sizeof(uint32_t)));
+    }
+    const uint32_t merge_blocks = ctx->num_cus * resident_blocks(reinterpret_cast<const void*>(&k_vcm_merge), 0u);
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
     if (count) fill(ctx, (uint32_t*)d_counts, 8);
@@ -746,12 +952,21 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
             hipLaunchKernelGGL(k_vcm_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, L,
                                b.verts, b.cnt, b.start, b.fill, b.phot);
         }
+        float4* fcol = merge_pass ? b.fcol : nullptr;
+        VcmQuery* queries = merge_pass ? b.queries : nullptr;
+        if (merge_pass) fill(ctx, b.qctr, 1);
         if (count)
-            hipLaunchKernelGGL((k_vcm_eye<true>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
-                               ctx->counters + 1, ctx->spill, cnt);
+            hipLaunchKernelGGL((k_vcm_eye<true>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum, fcol,
+                               queries, b.qctr, ctx->counters + 1, ctx->spill, cnt);
         else
-            hipLaunchKernelGGL((k_vcm_eye<false>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum,
-                               ctx->counters + 1, ctx->spill, cnt);
+            hipLaunchKernelGGL((k_vcm_eye<false>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum, fcol,
+                               queries, b.qctr, ctx->counters + 1, ctx->spill, cnt);
+        if (merge_pass) {
+            hipLaunchKernelGGL(k_vcm_merge, dim3(merge_blocks), block, 0, ctx->stream, sv, L, G, b.queries, b.qctr,
+                               b.fcol);
+            hipLaunchKernelGGL(k_vcm_resolve, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, b.fcol,
+                               b.splat, accum, (uint32_t)N, L.frame_num);
+        }
     }
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;
