@@ -19,6 +19,7 @@ struct VcmLaunch {
     float radius, eta_vcm, eta_vc, eta_vm;
     int32_t grid_res[3];
     float bmin[3], bmax[3];
+    float inv_ext[3];  // 1 / (bmax - bmin): the corrected estimator's grid (see k_vcm_scatter)
     float sky_col[3];
     uint32_t use_vm;
     uint32_t corrected;  // 0: the reference estimator, 1: corrected VCM
@@ -45,9 +46,10 @@ struct VcmVertex {
     uint4 f;   // material_idx, path_len, side, 0
 };
 
-// A photon of the corrected estimator's bucketed list, 64 B.
-struct VcmPhoton {
-    float4 a;  // pos.xyz, d_vcm
+// A photon of the corrected estimator's bucketed list: its position (and
+// d_vcm) in one array, 16 B, the rest in another, 48 B, so that the photons
+// a gather tests and rejects cost 16 B each.
+struct VcmPhotonRest {
     float4 b;  // wi.xyz (towards the light), d_vm
     float4 c;  // throughput.xyz, path_len (u32 bits)
     float4 d;  // n_s.xyz, 0
@@ -59,9 +61,9 @@ struct VcmPhoton {
 //  - cells: 10 * W * H hash buckets (utils.glsl:62-65).  Reference estimator:
 //    key = 1 + the largest vertex reference inserted into the bucket (the
 //    deterministic "last writer") and count = insertions.  Corrected
-//    estimator: count per bucket, its exclusive scan `start`, and `phot` =
-//    the photons themselves, bucket by bucket, so a gather streams each
-//    bucket's records contiguously.
+//    estimator: count per bucket, its exclusive scan `start`, and `ppos` /
+//    `prest` = the photons themselves, bucket by bucket, so a gather streams
+//    each bucket's records contiguously.
 //  - splat: light-tracing splats (float3 per pixel), read and cleared by the
 //    eye pass (by the resolve pass when merges are gathered separately).
 struct VcmBuffers {
@@ -71,7 +73,8 @@ struct VcmBuffers {
     uint32_t* count = nullptr;
     uint32_t* start = nullptr;
     uint32_t* fill = nullptr;
-    VcmPhoton* phot = nullptr;
+    float4* ppos = nullptr;  // pos.xyz, d_vcm
+    VcmPhotonRest* prest = nullptr;
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     float* splat = nullptr;

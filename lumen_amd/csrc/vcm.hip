@@ -1,6 +1,7 @@
 // vcm.hip — the VCM integrator on gfx950.
 //
-// A frame is a light pass, the photon grid, and an eye pass:
+// A frame is a light pass, the photon grid, an eye pass and, for the
+// corrected estimator with merging, a merge pass and a resolve:
 //   k_vcm_light   one light subpath per pixel slot (vcm_light.rgen:123-166,
 //                 vcm_commons.glsl:77-130, 302-474): stores its non-specular
 //                 vertices as 96-B records, splats camera connections with
@@ -9,12 +10,17 @@
 //                 reference inserted (atomicMax, a deterministic "last
 //                 writer") and the insertion count; corrected estimator: an
 //                 exclusive scan of the bucket counts and a scatter of every
-//                 photon reference (k_vcm_scatter);
+//                 photon (k_vcm_scatter: 16-B positions, 48-B rest);
 //   k_vcm_eye     one camera path per pixel (vcm_eye.rgen:42-98,
 //                 vcm_commons.glsl:477-631): emission, NEE, connections to the
-//                 vertices of one random light path, merging over the hash
-//                 grid; then the pixel's splats and the running mean.
-// All three are persistent: waves pull 64-item chunks from a counter.
+//                 vertices of one random light path; the reference estimator
+//                 merges in place and folds in the splats and the running
+//                 mean; the corrected one appends a merge query per
+//                 non-specular vertex and leaves the colour in fcol;
+//   k_vcm_merge   (corrected) one wave per merge query over the photons of
+//                 all its cells, sums added to fcol;
+//   k_vcm_resolve (corrected) fcol plus the splats into the running mean.
+// The light, eye and merge kernels are persistent.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -82,6 +88,16 @@ LT_DEV void grid_idx(const VcmLaunch& L, f3 p, int o[3]) {
     o[0] = (int)(qx * (float)L.grid_res[0]);
     o[1] = (int)(qy * (float)L.grid_res[1]);
     o[2] = (int)(qz * (float)L.grid_res[2]);
+}
+
+// The corrected estimator's grid: grid_idx with the extent's reciprocal.
+// Only the set of photons within the radius enters its sum, and any grid
+// whose cell index is monotone in the position finds each of them exactly
+// once, so the cheaper multiply changes nothing but bucket membership.
+LT_DEV void grid_idx_fast(const VcmLaunch& L, f3 p, int o[3]) {
+    o[0] = (int)((p.x - L.bmin[0]) * L.inv_ext[0] * (float)L.grid_res[0]);
+    o[1] = (int)((p.y - L.bmin[1]) * L.inv_ext[1] * (float)L.grid_res[1]);
+    o[2] = (int)((p.z - L.bmin[2]) * L.inv_ext[2] * (float)L.grid_res[2]);
 }
 
 template <bool COUNT>
@@ -246,8 +262,11 @@ LT_DEV uint32_t light_path(const SceneView& sv, const VcmLaunch& L, const Camera
     return path_idx;
 }
 
+#ifndef LT_VCM_LIGHT_WAVES
+#define LT_VCM_LIGHT_WAVES 4
+#endif
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_vcm_light(SceneView sv, VcmLaunch L, VcmVertex* __restrict__ verts,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_LIGHT_WAVES))) void k_vcm_light(SceneView sv, VcmLaunch L, VcmVertex* __restrict__ verts,
                                                            uint32_t* __restrict__ cnt, float* __restrict__ splat,
                                                            uint32_t* __restrict__ key, uint32_t* __restrict__ count,
                                                            uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
@@ -274,7 +293,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_vcm_light(SceneView sv, VcmLaun
             for (uint32_t i = 0; i < n; i++) {
                 const float4 a = lv[i].a;
                 int g[3];
-                grid_idx(L, f3{a.x, a.y, a.z}, g);
+                if (L.corrected)
+                    grid_idx_fast(L, f3{a.x, a.y, a.z}, g);
+                else
+                    grid_idx(L, f3{a.x, a.y, a.z}, g);
                 const uint32_t h = cell_hash(g[0], g[1], g[2], L.n_paths);
                 if (!L.corrected) atomicMax(&key[h], p * nv + i + 1u);
                 atomicAdd(&count[h], 1u);
@@ -301,7 +323,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_vcm_light(SceneView sv, VcmLaun
 // corrected estimator: every photon into its bucket's range
 __global__ void k_vcm_scatter(VcmLaunch L, const VcmVertex* __restrict__ verts, const uint32_t* __restrict__ cnt,
                               const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
-                              VcmPhoton* __restrict__ phot) {
+                              float4* __restrict__ ppos, VcmPhotonRest* __restrict__ prest) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.n_paths) return;
     const uint32_t nv = (uint32_t)L.max_depth + 1u;
@@ -310,10 +332,11 @@ __global__ void k_vcm_scatter(VcmLaunch L, const VcmVertex* __restrict__ verts, 
         const float4 a = v.a, b = v.b, c = v.c, e = v.e;
         const uint4 f = v.f;
         int g[3];
-        grid_idx(L, f3{a.x, a.y, a.z}, g);
+        grid_idx_fast(L, f3{a.x, a.y, a.z}, g);
         const uint32_t h = cell_hash(g[0], g[1], g[2], L.n_paths);
-        VcmPhoton& o = phot[start[h] + atomicAdd(&fill[h], 1u)];
-        o.a = a;
+        const uint32_t k = start[h] + atomicAdd(&fill[h], 1u);
+        ppos[k] = a;
+        VcmPhotonRest& o = prest[k];
         o.b = make_float4(-c.x, -c.y, -c.z, c.w);
         o.c = make_float4(e.x, e.y, e.z, __uint_as_float(f.y));
         o.d = make_float4(b.x, b.y, b.z, 0.0f);
@@ -324,7 +347,8 @@ __global__ void k_vcm_scatter(VcmLaunch L, const VcmVertex* __restrict__ verts, 
 struct Grid {
     const VcmVertex* verts;
     const uint32_t *cnt, *key, *count, *start;
-    const VcmPhoton* phot;
+    const float4* ppos;
+    const VcmPhotonRest* prest;
 };
 
 // vcm_commons.glsl:132-143
@@ -433,7 +457,9 @@ LT_DEV bool merge_one(const VcmLaunch& L, float dist_sqr, f3 pwi, f3 pthr, f3 pn
     return true;
 }
 
-// vcm_commons.glsl:251-296
+// vcm_commons.glsl:251-296, the reference estimator: one photon per bucket
+// (the deterministic "last writer") weighted by the bucket's count, the
+// result overwritten per cell
 LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, const Mat& mat, bool side,
                 const VState& cs, const Payload& pl, float norm) {
     f3 res = mk3(0.0f);
@@ -444,36 +470,16 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int z = lo[2]; z <= hi[2]; z++) {
                 const uint32_t h = cell_hash(x, y, z, L.n_paths);
-                if (!L.corrected) {
-                    const uint32_t cnt = G.count[h];
-                    if (cnt == 0u) continue;
-                    const LVtx v = load_lv(G.verts + (G.key[h] - 1u));
-                    const f3 pp = pl.pos - v.pos;
-                    const float dist_sqr = dot(pp, pp);
-                    if (dist_sqr > L.radius * L.radius) continue;
-                    f3 r;
-                    if (merge_one(L, dist_sqr, -v.wi, v.thr, v.n_s, v.d_vcm, v.d_vm, v.path_len, (float)(int)cnt, depth,
-                                  n_s, wo, mat, side, cs, norm, r))
-                        res = r;
-                } else {
-                    const uint32_t k1 = G.start[h + 1];
-                    for (uint32_t k = G.start[h]; k < k1; k++) {
-                        const VcmPhoton& ph = G.phot[k];
-                        const float4 a = ph.a;
-                        const f3 ppos = {a.x, a.y, a.z};
-                        const f3 pp = pl.pos - ppos;
-                        const float dist_sqr = dot(pp, pp);
-                        if (dist_sqr > L.radius * L.radius) continue;
-                        int g[3];
-                        grid_idx(L, ppos, g);
-                        if (g[0] != x || g[1] != y || g[2] != z) continue;  // another cell of the bucket
-                        const float4 b = ph.b, c = ph.c, d = ph.d;
-                        f3 r;
-                        if (merge_one(L, dist_sqr, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, a.w, b.w,
-                                      __float_as_uint(c.w), 1.0f, depth, n_s, wo, mat, side, cs, norm, r))
-                            res += r;
-                    }
-                }
+                const uint32_t cnt = G.count[h];
+                if (cnt == 0u) continue;
+                const LVtx v = load_lv(G.verts + (G.key[h] - 1u));
+                const f3 pp = pl.pos - v.pos;
+                const float dist_sqr = dot(pp, pp);
+                if (dist_sqr > L.radius * L.radius) continue;
+                f3 r;
+                if (merge_one(L, dist_sqr, -v.wi, v.thr, v.n_s, v.d_vcm, v.d_vm, v.path_len, (float)(int)cnt, depth,
+                              n_s, wo, mat, side, cs, norm, r))
+                    res = r;
             }
     return res;
 }
@@ -516,16 +522,19 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         const int depth = (int)(qe.z >> 1);
         const Mat mat = load_material(sv, qe.x, f2{qc.w, qd.w});
         int lo[3], hi[3];
-        grid_idx(L, qpos - mk3(L.radius), lo);
-        grid_idx(L, qpos + mk3(L.radius), hi);
+        grid_idx_fast(L, qpos - mk3(L.radius), lo);
+        grid_idx_fast(L, qpos + mk3(L.radius), hi);
         const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
         const int ncells = (hi[0] - lo[0] + 1) * ny * nz;
         f3 part = mk3(0.0f);
         for (int cbase = 0; cbase < ncells; cbase += 64) {
             const int ci = cbase + (int)lane;  // this lane's cell of the chunk: its photon range
             uint32_t st = 0u, cnt = 0u;
+            int cx = 0, cy = 0, cz = 0;
             if (ci < ncells) {
-                const int cx = lo[0] + ci / (ny * nz), cy = lo[1] + (ci / nz) % ny, cz = lo[2] + ci % nz;
+                cx = lo[0] + ci / (ny * nz);
+                cy = lo[1] + (ci / nz) % ny;
+                cz = lo[2] + ci % nz;
                 const uint32_t h = cell_hash(cx, cy, cz, L.n_paths);
                 st = Gr.start[h];
                 cnt = Gr.start[h + 1] - st;
@@ -545,18 +554,19 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                 }
                 j = j < 63 ? j : 63;
                 const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
+                const int jx = __shfl(cx, j), jy = __shfl(cy, j), jz = __shfl(cz, j);
                 if (t >= total) continue;
-                const VcmPhoton& ph = Gr.phot[sj + (t - ej)];
-                const float4 a = ph.a, b = ph.b, c = ph.c, d = ph.d;  // one 64-B record, loaded at once
+                const uint32_t k = sj + (t - ej);
+                const float4 a = Gr.ppos[k];
                 const f3 ppos = {a.x, a.y, a.z};
                 const f3 pp = qpos - ppos;
                 const float dist_sqr = dot(pp, pp);
                 if (dist_sqr > r2) continue;
-                const int cj = cbase + j;
                 int g[3];
-                grid_idx(L, ppos, g);
-                if (g[0] != lo[0] + cj / (ny * nz) || g[1] != lo[1] + (cj / nz) % ny || g[2] != lo[2] + cj % nz)
-                    continue;  // another cell of the bucket
+                grid_idx_fast(L, ppos, g);
+                if (g[0] != jx || g[1] != jy || g[2] != jz) continue;  // another cell of the bucket
+                const VcmPhotonRest& pr = Gr.prest[k];
+                const float4 b = pr.b, c = pr.c, d = pr.d;
                 f3 r;
                 if (merge_one(L, dist_sqr, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, a.w, b.w,
                               __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
@@ -876,7 +886,8 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
     LT_HIP(ctx, hipMalloc(&b.count, (cells + 1) * sizeof(uint32_t)));
     LT_HIP(ctx, hipMalloc(&b.start, (cells + 1) * sizeof(uint32_t)));
     LT_HIP(ctx, hipMalloc(&b.fill, cells * sizeof(uint32_t)));
-    LT_HIP(ctx, hipMalloc(&b.phot, N * nv * sizeof(VcmPhoton)));
+    LT_HIP(ctx, hipMalloc(&b.ppos, N * nv * sizeof(float4)));
+    LT_HIP(ctx, hipMalloc(&b.prest, N * nv * sizeof(VcmPhotonRest)));
     LT_HIP(ctx, hipMalloc(&b.splat, N * 3 * sizeof(float)));
     LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b.scan_bytes, b.count, b.start, (int)(cells + 1),
                                                  ctx->stream));
@@ -888,8 +899,8 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
 }
 
 void vcm_free(VcmBuffers& b) {
-    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.phot, b.scan_tmp, b.splat, b.fcol, b.queries,
-                    b.qctr};
+    void* ptrs[] = {b.verts, b.cnt, b.key, b.count, b.start, b.fill, b.ppos, b.prest, b.scan_tmp, b.splat, b.fcol,
+                    b.queries, b.qctr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = VcmBuffers{};
@@ -924,12 +935,13 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
     if (count) fill(ctx, (uint32_t*)d_counts, 8);
     const size_t N = b.n_paths, cells = 10 * N;
     const dim3 lgrid(light_blocks), egrid(eye_blocks), block(kTraceBlock);
-    const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.phot};
+    const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.ppos, b.prest};
     for (uint32_t f = 0; f < frames; f++) {
         VcmLaunch L = L0;
         L.frame_num = frame_num + f;
         vcm_frame_words(seed, L.frame_num, &L.light_word, &L.eye_word);
         vcm_radius_grid(L.bmin, L.bmax, cfg.radius_factor, cfg.alpha, L.frame_num, &L.radius, L.grid_res);
+        for (int k = 0; k < 3; k++) L.inv_ext[k] = 1.0f / (L.bmax[k] - L.bmin[k]);
         const float r2 = L.radius * L.radius;
         L.eta_vcm = PI * r2 * (float)N;
         L.eta_vc = 1.0f / L.eta_vcm;
@@ -950,7 +962,7 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
                                                          (int)(cells + 1), ctx->stream));
             fill(ctx, b.fill, cells);
             hipLaunchKernelGGL(k_vcm_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, L,
-                               b.verts, b.cnt, b.start, b.fill, b.phot);
+                               b.verts, b.cnt, b.start, b.fill, b.ppos, b.prest);
         }
         float4* fcol = merge_pass ? b.fcol : nullptr;
         VcmQuery* queries = merge_pass ? b.queries : nullptr;
