@@ -34,11 +34,24 @@ LT_DEV void trace_closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk
         pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
 }
 
-// pt_commons.glsl:3-42
-template <bool COUNT>
-LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f3 pos, bool side, f3 n_s, f3 wo,
-                               Payload& pl, const TraceStack& stk, Ctr& c) {
-    f3 res = mk3(0.0f);
+// pt_commons.glsl:3-42 in two halves, so that a bounce draws all its random
+// numbers and evaluates its BSDF before it traces anything and only this
+// record, not the material and the hit, lives across the traces.  The terms
+// are pre-scaled by the throughput and the light pick pdf of the bounce
+// (path.rgen:76-78).
+struct NeeRays {
+    f3 org;             // offset_ray2(pos, n_s): both rays start here
+    f3 wi, c_light;     // light sample: direction and contribution if unoccluded
+    float tmax;         // wi_len - EPS; 0 = no shadow ray
+    f3 wi_bsdf, c_bsdf; // area light: BSDF-sampled direction and its contribution before MIS
+    f3 pos, stale_n;    // the hit: pos, and its payload normal for a missed MIS ray
+    float pdf_light_a, pdf_bsdf;
+    uint32_t tri, inst; // the sampled light triangle
+    uint32_t flags;     // bit 0: MIS ray; bit 1: a missed MIS ray still matches (stale payload)
+};
+
+LT_DEV void nee_sample(const SceneView& sv, RNG& seed, const Mat& mat, const Payload& hp, bool side, f3 n_s, f3 wo,
+                       f3 scale, NeeRays& r) {
     f3 wi;
     float wi_len, pdf_light_w, pdf_light_a, cos_from_light;
     LightRecord rec;
@@ -49,8 +62,11 @@ LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f
     r4.y = rnd(seed);
     r4.z = rnd(seed);
     r4.w = rnd(seed);
-    const f3 Le = sample_light_Li(sv, r4, pos, pdf_light_w, wi, wi_len, pdf_light_a, cos_from_light, rec);
-    const f3 p = offset_ray2(pos, n_s);
+    const f3 Le = sample_light_Li(sv, r4, hp.pos, pdf_light_w, wi, wi_len, pdf_light_a, cos_from_light, rec);
+    r.org = offset_ray2(hp.pos, n_s);
+    r.wi = wi;
+    r.tmax = 0.0f;
+    r.flags = 0u;
     float bsdf_pdf, bsdf_rev;
     float cos_x = dot(n_s, wi);
     f3 f = eval_bsdf(n_s, wo, mat, side, wi, bsdf_pdf, bsdf_rev, false);
@@ -59,34 +75,69 @@ LT_DEV f3 uniform_sample_light(const SceneView& sv, RNG& seed, const Mat& mat, f
         const f3 contrib = mis * f * fabsf(cos_x) * Le / pdf_light_w;
         // the occlusion query only matters for a non-zero contribution
         if (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) {
-            if (COUNT) c.shadow++;
-            HitRec h;
-            if (!trace_bvh<true, COUNT>(sv, p, wi, 0.0f, wi_len - EPS, h, stk, c.nodes, c.tris)) res += contrib;
+            r.c_light = scale * contrib;
+            r.tmax = wi_len - EPS;
         }
     }
     if ((rec.flags & 7u) == LT_LIGHT_AREA) {
         const float r0 = rnd(seed), r1 = rnd(seed), r2 = rnd(seed);
         f = sample_bsdf(n_s, wo, mat, 1, side, wi, bsdf_pdf, cos_x, r0, r1, r2);
         if (bsdf_pdf != 0.0f) {
-            trace_closest<COUNT>(sv, p, wi, stk, pl, c);
-            if (pl.triangle_idx == rec.triangle_idx && pl.instance_idx == rec.instance_idx) {
-                const float wl = length(pl.pos - pos);
-                const float g = fabsf(dot(pl.n_s, -wi)) / (wl * wl);
-                const float mis = 1.0f / (1.0f + pdf_light_a / (g * bsdf_pdf));
-                res += f * mis * fabsf(cos_x) * Le / bsdf_pdf;
-            }
+            r.flags = 1u;
+            if (hp.triangle_idx == rec.triangle_idx && hp.instance_idx == rec.instance_idx) r.flags |= 2u;
+            r.wi_bsdf = wi;
+            r.c_bsdf = scale * (f * fabsf(cos_x) * Le / bsdf_pdf);
+            r.pos = hp.pos;
+            r.stale_n = hp.n_s;
+            r.pdf_light_a = pdf_light_a;
+            r.pdf_bsdf = bsdf_pdf;
+            r.tri = rec.triangle_idx;
+            r.inst = rec.instance_idx;
+        }
+    }
+}
+
+// The shadow ray and the MIS ray of nee_sample; the MIS ray's payload is the
+// bounce's hit where it misses (ray.rmiss:13 leaves the IDs, pos and normal).
+template <bool COUNT>
+LT_DEV f3 nee_trace(const SceneView& sv, const NeeRays& r, const TraceStack& stk, Ctr& c) {
+    f3 res = mk3(0.0f);
+    if (r.tmax != 0.0f) {
+        if (COUNT) c.shadow++;
+        HitRec h;
+        if (!trace_bvh<true, COUNT>(sv, r.org, r.wi, 0.0f, r.tmax, h, stk, c.nodes, c.tris)) res += r.c_light;
+    }
+    if (r.flags & 1u) {
+        Payload pl;
+        trace_closest<COUNT>(sv, r.org, r.wi_bsdf, stk, pl, c);
+        const bool match = pl.valid ? (pl.triangle_idx == r.tri && pl.instance_idx == r.inst) : (r.flags & 2u) != 0u;
+        if (match) {
+            const f3 hpos = pl.valid ? pl.pos : r.pos, hn = pl.valid ? pl.n_s : r.stale_n;
+            const float wl = length(hpos - r.pos);
+            const float g = fabsf(dot(hn, -r.wi_bsdf)) / (wl * wl);
+            const float mis = 1.0f / (1.0f + r.pdf_light_a / (g * r.pdf_bsdf));
+            res += r.c_bsdf * mis;
         }
     }
     return res;
 }
 
-// path.rgen:26-104; returns false when the NaN guard drops the sample
-template <bool COUNT>
-LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word,
-                        const TraceStack& stk, f3& col, Ctr& c) {
-    RNG seed = {x, y, frame_word, 0u};
-    const float r0 = rnd(seed);
-    const float r1 = rnd(seed);
+// The state of one path between bounces (path.rgen:26-100).  The hit
+// payload is not part of it: a bounce's closest-hit trace either fills every
+// field or misses and ends the path, so only the NEE MIS ray of the same
+// bounce sees a stale payload (ray.rmiss:13), which nee_sample records.
+struct PathState {
+    RNG seed;
+    f3 origin, direction, throughput, col;
+    uint32_t pix;
+    int depth;
+    bool last_specular;
+};
+
+LT_DEV void path_start(const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word, PathState& p) {
+    p.seed = {x, y, frame_word, 0u};
+    const float r0 = rnd(p.seed);
+    const float r1 = rnd(p.seed);
     const float jx = r0 - 0.5f, jy = r1 - 0.5f;
     const float ux = ((float)x + 0.5f + jx) / (float)L.width, uy = ((float)y + 0.5f + jy) / (float)L.height;
     const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
@@ -94,26 +145,33 @@ LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, ui
     const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
     const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
     const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
-    f3 origin = {o4.x, o4.y, o4.z};
-    f3 direction = {d4.x, d4.y, d4.z};
-    col = mk3(0.0f);
-    bool last_specular = false;
-    f3 throughput = mk3(1.0f);
-    Payload pl;
-    pl.valid = false;
-    pl.triangle_idx = pl.instance_idx = 0xFFFFFFFFu;
-    pl.pos = pl.n_s = pl.n_g = mk3(0.0f);
-    for (int depth = 0;; depth++) {
-        trace_closest<COUNT>(sv, origin, direction, stk, pl, c);
-        if (!pl.valid) {
-            if (depth > 0 || L.direct_lighting == 1) col += throughput * ld3(L.sky_col);
-            break;
+    p.origin = {o4.x, o4.y, o4.z};
+    p.direction = {d4.x, d4.y, d4.z};
+    p.col = mk3(0.0f);
+    p.throughput = mk3(1.0f);
+    p.last_specular = false;
+    p.depth = 0;
+}
+
+// one iteration of path.rgen:46-100; returns false when the path has ended.
+// The bounce draws its random numbers in the reference's order (NEE light
+// sample, NEE BSDF sample, continuation, Russian roulette), then traces its
+// NEE rays, whose terms use the throughput before the continuation.
+template <bool COUNT>
+LT_DEV bool path_bounce(const SceneView& sv, const PathLaunch& L, PathState& p, const TraceStack& stk, Ctr& c) {
+    NeeRays nee;
+    bool alive = true, has_nee = false;
+    {
+        Payload hp;
+        trace_closest<COUNT>(sv, p.origin, p.direction, stk, hp, c);
+        if (!hp.valid) {
+            if (p.depth > 0 || L.direct_lighting == 1) p.col += p.throughput * ld3(L.sky_col);
+            return false;
         }
-        const Payload hp = pl;
         const Mat hit_mat = load_material(sv, hp.material_idx, hp.uv);
-        if ((depth == 0 && L.direct_lighting == 1) || last_specular) col += throughput * hit_mat.emissive;
-        if (depth >= L.max_depth - 1) break;
-        const f3 wo = -direction;
+        if ((p.depth == 0 && L.direct_lighting == 1) || p.last_specular) p.col += p.throughput * hit_mat.emissive;
+        if (p.depth >= L.max_depth - 1) return false;
+        const f3 wo = -p.direction;
         f3 n_s = hp.n_s;
         bool side = true;
         f3 n_g = hp.n_g;
@@ -122,29 +180,46 @@ LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, ui
             n_s = -n_s;
             side = false;
         }
-        origin = offset_ray(hp.pos, n_g);
-        last_specular = is_specular(hit_mat);
-        if (!last_specular) {
+        p.origin = offset_ray(hp.pos, n_g);
+        p.last_specular = is_specular(hit_mat);
+        if (!p.last_specular && (p.depth > 0 || L.direct_lighting == 1)) {
             const float light_pick_pdf = 1.0f / (float)sv.total_light_triangle_cnt;
-            if (depth > 0 || L.direct_lighting == 1)
-                col += throughput * uniform_sample_light<COUNT>(sv, seed, hit_mat, hp.pos, side, n_s, wo, pl, stk, c) /
-                       light_pick_pdf;
+            nee_sample(sv, p.seed, hit_mat, hp, side, n_s, wo, p.throughput / light_pick_pdf, nee);
+            has_nee = true;
         }
         float pdf, cos_theta;
-        const float s0 = rnd(seed), s1 = rnd(seed), s2 = rnd(seed);
-        const f3 f = sample_bsdf(n_s, wo, hit_mat, 1, side, direction, pdf, cos_theta, s0, s1, s2);
-        if (pdf == 0.0f) break;
-        throughput *= f * fabsf(cos_theta) / pdf;
-        float rr_scale = 1.0f;
-        if (hit_mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) rr_scale *= side ? 1.0f / hit_mat.ior : hit_mat.ior;
-        if (depth > RR_MIN_DEPTH) {
-            const float rr_prob = fminf(0.95f, luminance(throughput) * rr_scale);
-            if (rr_prob == 0.0f || rr_prob < rnd(seed))
-                break;
-            else
-                throughput /= rr_prob;
+        const float s0 = rnd(p.seed), s1 = rnd(p.seed), s2 = rnd(p.seed);
+        const f3 f = sample_bsdf(n_s, wo, hit_mat, 1, side, p.direction, pdf, cos_theta, s0, s1, s2);
+        if (pdf == 0.0f) {
+            alive = false;
+        } else {
+            const f3 thr = p.throughput * (f * fabsf(cos_theta) / pdf);
+            float rr_scale = 1.0f;
+            if (hit_mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) rr_scale *= side ? 1.0f / hit_mat.ior : hit_mat.ior;
+            if (p.depth > RR_MIN_DEPTH) {
+                const float rr_prob = fminf(0.95f, luminance(thr) * rr_scale);
+                if (rr_prob == 0.0f || rr_prob < rnd(p.seed)) alive = false;
+                else p.throughput = thr / rr_prob;
+            } else {
+                p.throughput = thr;
+            }
         }
     }
+    if (has_nee) p.col += nee_trace<COUNT>(sv, nee, stk, c);
+    p.depth++;
+    return alive;
+}
+
+// One path of pixel (x, y) (path.rgen:26-104); false when the NaN guard
+// drops the sample.
+template <bool COUNT>
+LT_DEV bool path_sample(const SceneView& sv, const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word,
+                        const TraceStack& stk, f3& col, Ctr& c) {
+    PathState p;
+    path_start(L, x, y, frame_word, p);
+    while (path_bounce<COUNT>(sv, L, p, stk, c)) {
+    }
+    col = p.col;
     return !isnan(luminance(col));
 }
 
@@ -218,83 +293,6 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
 // of uneven length (Russian roulette, escapes) never leave lanes idle until
 // the longest path of a tile ends.  Per path the arithmetic, the RNG stream
 // and the result are those of path_sample.
-struct PathState {
-    RNG seed;
-    f3 origin, direction, throughput, col;
-    Payload pl;  // persists across the path's traces (ray.rmiss:13)
-    uint32_t pix;
-    int depth;
-    bool last_specular;
-};
-
-LT_DEV void path_start(const PathLaunch& L, uint32_t x, uint32_t y, uint32_t frame_word, PathState& p) {
-    p.seed = {x, y, frame_word, 0u};
-    const float r0 = rnd(p.seed);
-    const float r1 = rnd(p.seed);
-    const float jx = r0 - 0.5f, jy = r1 - 0.5f;
-    const float ux = ((float)x + 0.5f + jx) / (float)L.width, uy = ((float)y + 0.5f + jy) / (float)L.height;
-    const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
-    const f4 o4 = mul4(L.inv_view, f4{0.0f, 0.0f, 0.0f, 1.0f});
-    const f4 tg = mul4(L.inv_projection, f4{dx, dy, 1.0f, 1.0f});
-    const f3 tn = normalize(mk3(tg.x, tg.y, tg.z));
-    const f4 d4 = mul4(L.inv_view, f4{tn.x, tn.y, tn.z, 0.0f});
-    p.origin = {o4.x, o4.y, o4.z};
-    p.direction = {d4.x, d4.y, d4.z};
-    p.col = mk3(0.0f);
-    p.throughput = mk3(1.0f);
-    p.last_specular = false;
-    p.depth = 0;
-    p.pl.valid = false;
-    p.pl.triangle_idx = p.pl.instance_idx = 0xFFFFFFFFu;
-    p.pl.pos = p.pl.n_s = p.pl.n_g = mk3(0.0f);
-}
-
-// one iteration of path.rgen:46-100; returns false when the path has ended
-template <bool COUNT>
-LT_DEV bool path_bounce(const SceneView& sv, const PathLaunch& L, PathState& p, const TraceStack& stk, Ctr& c) {
-    trace_closest<COUNT>(sv, p.origin, p.direction, stk, p.pl, c);
-    if (!p.pl.valid) {
-        if (p.depth > 0 || L.direct_lighting == 1) p.col += p.throughput * ld3(L.sky_col);
-        return false;
-    }
-    const Payload hp = p.pl;
-    const Mat hit_mat = load_material(sv, hp.material_idx, hp.uv);
-    if ((p.depth == 0 && L.direct_lighting == 1) || p.last_specular) p.col += p.throughput * hit_mat.emissive;
-    if (p.depth >= L.max_depth - 1) return false;
-    const f3 wo = -p.direction;
-    f3 n_s = hp.n_s;
-    bool side = true;
-    f3 n_g = hp.n_g;
-    if (dot(hp.n_g, wo) < 0.0f) n_g = -n_g;
-    if (dot(n_g, hp.n_s) < 0.0f) {
-        n_s = -n_s;
-        side = false;
-    }
-    p.origin = offset_ray(hp.pos, n_g);
-    p.last_specular = is_specular(hit_mat);
-    if (!p.last_specular) {
-        const float light_pick_pdf = 1.0f / (float)sv.total_light_triangle_cnt;
-        if (p.depth > 0 || L.direct_lighting == 1)
-            p.col += p.throughput *
-                     uniform_sample_light<COUNT>(sv, p.seed, hit_mat, hp.pos, side, n_s, wo, p.pl, stk, c) /
-                     light_pick_pdf;
-    }
-    float pdf, cos_theta;
-    const float s0 = rnd(p.seed), s1 = rnd(p.seed), s2 = rnd(p.seed);
-    const f3 f = sample_bsdf(n_s, wo, hit_mat, 1, side, p.direction, pdf, cos_theta, s0, s1, s2);
-    if (pdf == 0.0f) return false;
-    p.throughput *= f * fabsf(cos_theta) / pdf;
-    float rr_scale = 1.0f;
-    if (hit_mat.bsdf_props & LT_BSDF_FLAG_TRANSMISSION) rr_scale *= side ? 1.0f / hit_mat.ior : hit_mat.ior;
-    if (p.depth > RR_MIN_DEPTH) {
-        const float rr_prob = fminf(0.95f, luminance(p.throughput) * rr_scale);
-        if (rr_prob == 0.0f || rr_prob < rnd(p.seed)) return false;
-        p.throughput /= rr_prob;
-    }
-    p.depth++;
-    return true;
-}
-
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_PATH_MEGA_WAVES))) void k_path_regen(
     SceneView sv, PathLaunch L, float4* __restrict__ accum, uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
