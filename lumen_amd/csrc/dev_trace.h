@@ -238,7 +238,10 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         const float margin = far_kz * (1.0f / 65536.0f);
         const float lo_lim = tmin - margin;
         const float hi_lim = tcur + margin;
-        uint32_t hit_int = 0, hit_leaf = 0;
+        // An empty slot (planes lo > hi) carries neither an internal bit nor a
+        // leaf count, so a box test that lets one through (possible only
+        // within the rounding margin) selects nothing.
+        uint32_t hm = 0;
         float tn_c[8];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
@@ -252,16 +255,17 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
             float t1 = fminf(fminf(fmaf(lane16(qfx), ax, bfx), fmaf(lane16(qfy), ay, bfy)), fmaf(lane16(qfz), az, bfz));
             t0 = fmaxf(t0, lo_lim);
             t1 = fminf(t1, hi_lim);
-            const uint32_t wlo = wd == 0 ? q0.x : (wd == 1 ? q0.y : (wd == 2 ? q0.z : q0.w));
-            const uint32_t whi = wd == 0 ? q3.x : (wd == 1 ? q3.y : (wd == 2 ? q3.z : q3.w));
-            const bool valid = ((wlo >> sh) & 0xFFFFu) <= ((whi >> sh) & 0xFFFFu);  // empty slots: lo > hi
-            const bool h = (t0 <= t1) && valid;
             tn_c[c] = t0;
-            const uint32_t meta = ((c < 4 ? meta_lo : meta_hi) >> ((c & 3) * 8)) & 0xFF;
-            const bool internal = (imask >> c) & 1u;
-            if (h && internal) hit_int |= 1u << c;
-            if (h && !internal && (meta & 7u) != 0) hit_leaf |= 1u << c;
+            hm |= (t0 <= t1 ? 1u : 0u) << c;
         }
+        // slots whose leaf holds triangles: a non-zero count (meta & 7) per byte
+        const uint32_t nz_lo = ((meta_lo & 0x07070707u) + 0x7F7F7F7Fu) & 0x80808080u;
+        const uint32_t nz_hi = ((meta_hi & 0x07070707u) + 0x7F7F7F7Fu) & 0x80808080u;
+        const uint32_t leafmask = ((nz_lo >> 7) & 1u) | ((nz_lo >> 14) & 2u) | ((nz_lo >> 21) & 4u) |
+                                  ((nz_lo >> 28) & 8u) | ((nz_hi >> 3) & 0x10u) | ((nz_hi >> 10) & 0x20u) |
+                                  ((nz_hi >> 17) & 0x40u) | ((nz_hi >> 24) & 0x80u);
+        const uint32_t hit_int = hm & imask;
+        uint32_t hit_leaf = hm & ~imask & leafmask;
         // leaves first: shrink tcur before ordering the internal children
         while (hit_leaf) {
             const int c = __builtin_ctz(hit_leaf);
