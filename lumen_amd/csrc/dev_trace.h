@@ -198,12 +198,14 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         const ltb::Node8* np = sv.nodes + ni;
         const float4 w0 = *((const float4*)np + 0);
         const uint4 w1 = *((const uint4*)np + 1);
-        const uint4 q0 = *((const uint4*)np + 2);
-        const uint4 q1 = *((const uint4*)np + 3);
-        const uint4 q2 = *((const uint4*)np + 4);
-        const uint4 q3 = *((const uint4*)np + 5);
-        const uint4 q4 = *((const uint4*)np + 6);
-        const uint4 q5 = *((const uint4*)np + 7);
+        // near / far planes by the ray octant: the plane vectors sit at
+        // 16-byte slots 2..4 (qlo x, y, z) and 5..7 (qhi), so the octant picks
+        // the load address instead of selecting between loaded words
+        const bool negx = rp.inv.x < 0.0f, negy = rp.inv.y < 0.0f, negz = rp.inv.z < 0.0f;
+        const uint4* q16 = (const uint4*)np;
+        const uint4 qnx = q16[negx ? 5 : 2], qfx = q16[negx ? 2 : 5];
+        const uint4 qny = q16[negy ? 6 : 3], qfy = q16[negy ? 3 : 6];
+        const uint4 qnz = q16[negz ? 7 : 4], qfz = q16[negz ? 4 : 7];
         const uint32_t ew = __float_as_uint(w0.w);  // e.x, e.y, e.z, imask
         const int ex = (int)(int8_t)(ew & 0xFF), ey = (int)(int8_t)((ew >> 8) & 0xFF),
                   ez = (int)(int8_t)((ew >> 16) & 0xFF);
@@ -222,11 +224,6 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         const float Ey = kSlabErr * fmaf(65535.0f, fabsf(ay), fabsf(by));
         const float Ez = kSlabErr * fmaf(65535.0f, fabsf(az), fabsf(bz));
         const float bnx = bx - Ex, bfx = bx + Ex, bny = by - Ey, bfy = by + Ey, bnz = bz - Ez, bfz = bz + Ez;
-        // near / far planes by the ray octant
-        const bool negx = rp.inv.x < 0.0f, negy = rp.inv.y < 0.0f, negz = rp.inv.z < 0.0f;
-        const uint4 qnx = negx ? q3 : q0, qfx = negx ? q0 : q3;
-        const uint4 qny = negy ? q4 : q1, qfy = negy ? q1 : q4;
-        const uint4 qnz = negz ? q5 : q2, qfz = negz ? q2 : q5;
         // The watertight t of a triangle carries an absolute error of a few ulp
         // of the largest sheared z of its vertices, i.e. of the farthest t of
         // the enclosing box along the dominant axis kz.  Children are kept if
