@@ -85,6 +85,26 @@ __global__ void k_make_tris(const lt_vertex* __restrict__ vtx, const uint32_t* _
     }
 }
 
+// fp16 bit patterns of non-negative plane values, rounded outward (the
+// largest fp16 <= x / the smallest fp16 >= x, both within [0, 65504]): the
+// double -> float -> fp16 conversion lands on one of the two fp16 values
+// around x, and one step along the (monotone) bit pattern fixes the side.
+__device__ __forceinline__ float f16_value(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+__device__ __forceinline__ uint16_t f16_floor(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 65504.0) return 0x7BFF;
+    uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)(float)x);
+    if ((double)f16_value(h) > x) h -= 1;
+    return h;
+}
+__device__ __forceinline__ uint16_t f16_ceil(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 65504.0) return 0x7BFF;
+    uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)(float)x);
+    if ((double)f16_value(h) < x) h += 1;
+    return h;
+}
+
 __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
     v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -291,14 +311,14 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
     for (int k = 0; k < 3; k++) {
         nd.p[k] = nb.lo[k];
         double extent = (double)nb.hi[k] - (double)nb.lo[k];
-        double s = extent / 65535.0;
+        double s = extent / 65504.0;
         int e;
         if (s <= 0.0) {
             e = -100;
         } else {
             double m = frexp(s, &e);  // s = m * 2^e, m in [0.5, 1)
             if (m == 0.5) e -= 1;
-            if (ldexp(65535.0, e) < extent) e += 1;
+            if (ldexp(65504.0, e) < extent) e += 1;
             if (e < -100) e = -100;
         }
         ex[k] = e;
@@ -312,17 +332,15 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
         if (c >= nc) {
             nd.meta[c] = 0;
             for (int k = 0; k < 3; k++) {
-                nd.qlo[k][c] = 0xFFFF;  // empty: lo > hi, never hit
+                nd.qlo[k][c] = 0x7BFF;  // empty: lo (65504) > hi (0)
                 nd.qhi[k][c] = 0;
             }
             continue;
         }
         for (int k = 0; k < 3; k++) {
             double inv = ldexp(1.0, -ex[k]);
-            double ql = floor(((double)cb[c].lo[k] - (double)nd.p[k]) * inv);
-            double qh = ceil(((double)cb[c].hi[k] - (double)nd.p[k]) * inv);
-            nd.qlo[k][c] = (uint16_t)fmin(fmax(ql, 0.0), 65535.0);
-            nd.qhi[k][c] = (uint16_t)fmin(fmax(qh, 0.0), 65535.0);
+            nd.qlo[k][c] = f16_floor(((double)cb[c].lo[k] - (double)nd.p[k]) * inv);
+            nd.qhi[k][c] = f16_ceil(((double)cb[c].hi[k] - (double)nd.p[k]) * inv);
         }
         if (is_leaf[c]) {
             uint32_t first, cnt;

@@ -2,9 +2,12 @@
 // (shared by the builder and the traversal kernels).
 //
 // BVH8 node: exactly 128 bytes (one gfx950 L2 cache line), 128-B aligned.
-// Child boxes are quantised to 16 bits per plane against a per-node origin p
-// and power-of-two scales 2^e, so decoding p + q * 2^e is exact in the
-// product and monotone in the sum: a decoded box always contains the child.
+// Child boxes are stored per plane as fp16 values q in [0, 65504] against a
+// per-node origin p and power-of-two scales 2^e (lower planes rounded down,
+// upper planes rounded up), so decoding p + q * 2^e is exact in the product
+// and monotone in the sum: a decoded box always contains the child.  fp16
+// rather than 16-bit integers lets the slab test convert and multiply-add a
+// plane in one v_fma_mix_f32.  Empty slots: lower 65504, upper 0.
 //
 //   bytes   0..11  p.x p.y p.z         origin (min corner of the node box)
 //          12..14  e.x e.y e.z         int8 exponents
@@ -13,8 +16,8 @@
 //          20..23  tri_base            index of the first triangle record
 //          24..31  meta[8]             internal: slot (child_base + meta)
 //                                      leaf: (tri offset << 3) | count, 0 = empty
-//          32..79  qlo[3][8]           uint16 lower planes per axis
-//          80..127 qhi[3][8]           uint16 upper planes per axis
+//          32..79  qlo[3][8]           fp16 lower planes per axis
+//          80..127 qhi[3][8]           fp16 upper planes per axis
 //
 // Triangle record: 48 bytes = v0, v1, v2 (world space, 36 B) + global
 // triangle index + mesh-local primitive index + mesh (instance) index.

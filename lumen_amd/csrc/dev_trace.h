@@ -98,7 +98,8 @@ struct HitRec {
 
 // Relative bound (2^-21) on the rounding of a child slab distance q * a + b:
 // a = 2^e * inv and b = (p - o) * inv each carry one rounding, inv one more
-// and the fused multiply-add one, i.e. |error| < 4 eps (65535 |a| + |b|).
+// and the fused multiply-add one, i.e. |error| < 4 eps (65535 |a| + |b|)
+// (planes are at most 65504).
 constexpr float kSlabErr = 4.76837158e-7f;
 
 // LDS-qualified so that stack accesses compile to ds_read / ds_write rather
@@ -244,9 +245,9 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         for (int c = 0; c < 8; c++) {
             const int sh = (c & 1) * 16;
             const int wd = c >> 1;
-            auto lane16 = [&](const uint4& q) {
+            auto lane16 = [&](const uint4& q) {  // the fp16 plane; the fma below takes it as is
                 const uint32_t w = wd == 0 ? q.x : (wd == 1 ? q.y : (wd == 2 ? q.z : q.w));
-                return (float)((w >> sh) & 0xFFFFu);
+                return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> sh));
             };
             float t0 = fmaxf(fmaxf(fmaf(lane16(qnx), ax, bnx), fmaf(lane16(qny), ay, bny)), fmaf(lane16(qnz), az, bnz));
             float t1 = fminf(fminf(fmaf(lane16(qfx), ax, bfx), fmaf(lane16(qfy), ay, bfy)), fmaf(lane16(qfz), az, bfz));

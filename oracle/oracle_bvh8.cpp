@@ -28,6 +28,13 @@ static_assert(sizeof(TriRech) == 48, "tri layout");
 
 static const float kSlabErr = 4.76837158e-7f;  // 2^-21
 
+// value of an fp16 bit pattern (the node planes); exact
+static float f16v(uint16_t h) {
+    const int ex = (h >> 10) & 0x1F, man = h & 0x3FF;
+    const float mag = ex == 0 ? ldexpf((float)man, -24) : ldexpf((float)(man | 0x400), ex - 25);
+    return (h & 0x8000) ? -mag : mag;
+}
+
 }  // namespace ora
 
 using namespace ora;
@@ -79,14 +86,14 @@ extern "C" void ora_bvh8_trace(const void* nodes_v, uint32_t n_nodes, const void
                 float qn[3], qf[3];
                 for (int k = 0; k < 3; k++) {
                     bool neg = inv[k] < 0.0f;  // ray octant picks the near plane
-                    qn[k] = (float)(neg ? nd.qhi[k][c] : nd.qlo[k][c]);
-                    qf[k] = (float)(neg ? nd.qlo[k][c] : nd.qhi[k][c]);
+                    qn[k] = f16v(neg ? nd.qhi[k][c] : nd.qlo[k][c]);
+                    qf[k] = f16v(neg ? nd.qlo[k][c] : nd.qhi[k][c]);
                 }
                 float t0 = fmaxf(fmaxf(fmaf(qn[0], a[0], bn[0]), fmaf(qn[1], a[1], bn[1])), fmaf(qn[2], a[2], bn[2]));
                 float t1 = fminf(fminf(fmaf(qf[0], a[0], bf[0]), fmaf(qf[1], a[1], bf[1])), fmaf(qf[2], a[2], bf[2]));
                 t0 = fmaxf(t0, tmin - margin);
                 t1 = fminf(t1, tcur + margin);
-                bool h = (t0 <= t1) && (nd.qlo[0][c] <= nd.qhi[0][c]);
+                bool h = t0 <= t1;  // an empty slot carries no internal bit and no count
                 tn[c] = t0;
                 bool internal = (nd.imask >> c) & 1u;
                 if (h && internal) hit_int |= 1u << c;
