@@ -196,17 +196,22 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         }
         s.next = kNoNode;
         if (COUNT) n_nodes++;
-        const ltb::Node8* np = sv.nodes + ni;
-        const float4 w0 = *((const float4*)np + 0);
-        const uint4 w1 = *((const uint4*)np + 1);
+        // 32-bit byte offsets from the node array (scalar base + vector offset
+        // addressing; the builder caps a BVH at 2^24 nodes)
+        const char* nbase = (const char*)sv.nodes;
+        const uint32_t noff = ni * 128u;
+        const float4 w0 = *(const float4*)(nbase + noff);
+        const uint4 w1 = *(const uint4*)(nbase + (noff + 16u));
         // near / far planes by the ray octant: the plane vectors sit at
         // 16-byte slots 2..4 (qlo x, y, z) and 5..7 (qhi), so the octant picks
         // the load address instead of selecting between loaded words
         const bool negx = rp.inv.x < 0.0f, negy = rp.inv.y < 0.0f, negz = rp.inv.z < 0.0f;
-        const uint4* q16 = (const uint4*)np;
-        const uint4 qnx = q16[negx ? 5 : 2], qfx = q16[negx ? 2 : 5];
-        const uint4 qny = q16[negy ? 6 : 3], qfy = q16[negy ? 3 : 6];
-        const uint4 qnz = q16[negz ? 7 : 4], qfz = q16[negz ? 4 : 7];
+        const uint4 qnx = *(const uint4*)(nbase + (noff + (negx ? 80u : 32u)));
+        const uint4 qfx = *(const uint4*)(nbase + (noff + (negx ? 32u : 80u)));
+        const uint4 qny = *(const uint4*)(nbase + (noff + (negy ? 96u : 48u)));
+        const uint4 qfy = *(const uint4*)(nbase + (noff + (negy ? 48u : 96u)));
+        const uint4 qnz = *(const uint4*)(nbase + (noff + (negz ? 112u : 64u)));
+        const uint4 qfz = *(const uint4*)(nbase + (noff + (negz ? 64u : 112u)));
         const uint32_t ew = __float_as_uint(w0.w);  // e.x, e.y, e.z, imask
         const int ex = (int)(int8_t)(ew & 0xFF), ey = (int)(int8_t)((ew >> 8) & 0xFF),
                   ez = (int)(int8_t)((ew >> 16) & 0xFF);
@@ -217,9 +222,10 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         // a = 2^e / d and b = (p - o) / d.  E bounds the rounding of that
         // evaluation against the exact (p + q 2^e - o) / d, so
         // [t_near - E, t_far + E] contains every child's true slab interval.
-        const float ax = __int_as_float((ex + 127) << 23) * rp.inv.x;
-        const float ay = __int_as_float((ey + 127) << 23) * rp.inv.y;
-        const float az = __int_as_float((ez + 127) << 23) * rp.inv.z;
+        // 2^e * inv, exact (a power-of-two scaling), as one ldexp
+        const float ax = ldexpf(rp.inv.x, ex);
+        const float ay = ldexpf(rp.inv.y, ey);
+        const float az = ldexpf(rp.inv.z, ez);
         const float bx = (w0.x - o.x) * rp.inv.x, by = (w0.y - o.y) * rp.inv.y, bz = (w0.z - o.z) * rp.inv.z;
         const float Ex = kSlabErr * fmaf(65535.0f, fabsf(ax), fabsf(bx));
         const float Ey = kSlabErr * fmaf(65535.0f, fabsf(ay), fabsf(by));
@@ -236,9 +242,6 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         const float margin = far_kz * (1.0f / 65536.0f);
         const float lo_lim = tmin - margin;
         const float hi_lim = tcur + margin;
-        // An empty slot (planes lo > hi) carries neither an internal bit nor a
-        // leaf count, so a box test that lets one through (possible only
-        // within the rounding margin) selects nothing.
         uint32_t hm = 0;
         float tn_c[8];
 #pragma unroll
@@ -256,14 +259,10 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
             tn_c[c] = t0;
             hm |= (t0 <= t1 ? 1u : 0u) << c;
         }
-        // slots whose leaf holds triangles: a non-zero count (meta & 7) per byte
-        const uint32_t nz_lo = ((meta_lo & 0x07070707u) + 0x7F7F7F7Fu) & 0x80808080u;
-        const uint32_t nz_hi = ((meta_hi & 0x07070707u) + 0x7F7F7F7Fu) & 0x80808080u;
-        const uint32_t leafmask = ((nz_lo >> 7) & 1u) | ((nz_lo >> 14) & 2u) | ((nz_lo >> 21) & 4u) |
-                                  ((nz_lo >> 28) & 8u) | ((nz_hi >> 3) & 0x10u) | ((nz_hi >> 10) & 0x20u) |
-                                  ((nz_hi >> 17) & 0x40u) | ((nz_hi >> 24) & 0x80u);
         const uint32_t hit_int = hm & imask;
-        uint32_t hit_leaf = hm & ~imask & leafmask;
+        // hit slots that are not internal: leaves, and (rarely, within the
+        // rounding margin) empty slots, whose count of 0 tests nothing
+        uint32_t hit_leaf = hm & ~imask;
         // leaves first: shrink tcur before ordering the internal children
         while (hit_leaf) {
             const int c = __builtin_ctz(hit_leaf);
