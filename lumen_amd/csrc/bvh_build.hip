@@ -278,16 +278,30 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
             ch[nc++] = right[b];
         }
     }
-    // classify
+    // classify; internal children take the first slots (in their order), so
+    // the traversal finds an internal child's node at child_base + slot
     uint32_t n_int = 0, n_leaf_tris = 0;
     bool is_leaf[8];
-    for (int c = 0; c < nc; c++) {
-        uint32_t cnt = count_of(ch[c], rlo, rhi);
-        is_leaf[c] = (ch[c] & 0x80000000u) || cnt <= max_leaf;
-        if (is_leaf[c])
-            n_leaf_tris += cnt;
-        else
-            n_int++;
+    {
+        uint32_t ord[8];
+        int ni = 0, nl = 0;
+        for (int c = 0; c < nc; c++) {
+            const uint32_t cnt = count_of(ch[c], rlo, rhi);
+            if ((ch[c] & 0x80000000u) || cnt <= max_leaf) {
+                n_leaf_tris += cnt;
+            } else {
+                ord[ni++] = ch[c];
+                n_int++;
+            }
+        }
+        for (int c = 0; c < nc; c++) {
+            const uint32_t cnt = count_of(ch[c], rlo, rhi);
+            if ((ch[c] & 0x80000000u) || cnt <= max_leaf) ord[ni + nl++] = ch[c];
+        }
+        for (int c = 0; c < nc; c++) {
+            ch[c] = ord[c];
+            is_leaf[c] = c >= (int)n_int;
+        }
     }
     uint32_t child_base = n_int ? atomicAdd(&ctr[0], n_int) : 0u;
     uint32_t tri_base = n_leaf_tris ? atomicAdd(&ctr[1], n_leaf_tris) : 0u;

@@ -11,10 +11,12 @@
 //
 //   bytes   0..11  p.x p.y p.z         origin (min corner of the node box)
 //          12..14  e.x e.y e.z         int8 exponents
-//          15      imask               bit i: child i is an internal node
+//          15      imask               bit i: child i is an internal node; the
+//                                      internal children fill the low slots, so
+//                                      imask = 2^n_internal - 1
 //          16..19  child_base          node index of the first internal child
 //          20..23  tri_base            index of the first triangle record
-//          24..31  meta[8]             internal: slot (child_base + meta)
+//          24..31  meta[8]             internal: its slot (node child_base + slot)
 //                                      leaf: (tri offset << 3) | count, 0 = empty
 //          32..79  qlo[3][8]           fp16 lower planes per axis
 //          80..127 qhi[3][8]           fp16 upper planes per axis

@@ -134,10 +134,10 @@ LT_DEV void stk_set_top(const TraceStack& s, uint32_t sp, uint32_t v) {
 }
 
 // A stack entry is a group of sibling internal nodes still to visit:
-// (child_base << 8) | mask over their ranks (child slot c is node
-// child_base + rank(c), rank = its index among the node's internal
-// children).  One entry per tree level instead of one per child; the
-// group's nodes are visited from the highest rank down.
+// (child_base << 8) | mask over their slots (the internal children fill the
+// low slots, child slot c is node child_base + c).  One entry per tree level
+// instead of one per child; the group's nodes are visited from the highest
+// slot down.
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 // The traversal of one query as a state machine, so that a lane can take a
@@ -309,19 +309,11 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
                     nearest = c;
                 }
             // the nearest is visited next; the others wait as one group and
-            // are visited from the highest slot down (internal children
-            // sit at child_base + rank in slot order)
-            uint32_t rest = hit_int & ~(1u << nearest);
-            if (rest) {
-                uint32_t ranks = 0;
-                while (rest) {
-                    const int c = __builtin_ctz(rest);
-                    rest &= rest - 1;
-                    ranks |= 1u << __popc(imask & ((1u << c) - 1u));
-                }
-                stk_push(stk, sp, (child_base << 8) | ranks);
-            }
-            s.next = child_base + (uint32_t)__popc(imask & ((1u << nearest) - 1u));
+            // are visited from the highest slot down (internal children fill
+            // the low slots: child c is node child_base + c)
+            const uint32_t rest = hit_int & ~(1u << nearest);
+            if (rest) stk_push(stk, sp, (child_base << 8) | rest);
+            s.next = child_base + (uint32_t)nearest;
         }
     }
     return s.next == kNoNode && sp == 0;
