@@ -403,7 +403,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
 template <bool BACKWARD>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_ADJ_WAVES))) void k_adjoint(SceneView sv, AdjLaunch L, AdjBuffers B,
                                                          uint32_t* __restrict__ work, uint32_t* __restrict__ spill) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;

@@ -692,7 +692,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     SceneView sv, BdptLaunch L, float* __restrict__ verts, uint32_t* __restrict__ hdr, uint32_t slots, uint32_t tile0,
     uint32_t n_tiles, uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
     unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;
@@ -767,7 +767,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                                                               uint32_t* __restrict__ work,
                                                               uint32_t* __restrict__ spill,
                                                               unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;

@@ -51,6 +51,8 @@ struct alignas(16) TriRec {
 static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
 
 constexpr int kLdsStack = 16;     // stack entries per thread kept in LDS
+constexpr int kLeafQueue = 16;    // queued leaf entries per thread (LDS)
+constexpr int kLdsWords = kLdsStack + kLeafQueue;  // LDS words per thread of a traversal
 constexpr int kStackTotal = 192;  // LDS + global spill entries per thread
 constexpr int kMaxLeafTris = 7;   // count field is 3 bits
 

@@ -108,8 +108,10 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 struct TraceStack {
     lds_u32* lds;     // this thread's column: entry i at lds[i * kTraceBlock]
+    lds_u32* q;       // its leaf queue, after the kLdsStack stack entries
     uint32_t* spill;  // this thread's spill area
-    LT_DEV TraceStack(uint32_t* lds_col, uint32_t* spill_area) : lds((lds_u32*)lds_col), spill(spill_area) {}
+    LT_DEV TraceStack(uint32_t* lds_col, uint32_t* spill_area)
+        : lds((lds_u32*)lds_col), q((lds_u32*)lds_col + ltb::kLdsStack * kTraceBlock), spill(spill_area) {}
 };
 
 LT_DEV void stk_push(const TraceStack& s, uint32_t& sp, uint32_t v) {
@@ -133,6 +135,11 @@ LT_DEV void stk_set_top(const TraceStack& s, uint32_t sp, uint32_t v) {
         s.spill[sp - 1 - ltb::kLdsStack] = v;
 }
 
+// a lane takes its next node while at most this many leaf entries are queued
+#ifndef LT_NODE_PEND_MAX
+#define LT_NODE_PEND_MAX (ltb::kLeafQueue - 8)
+#endif
+
 // A stack entry is a group of sibling internal nodes still to visit:
 // (child_base << 8) | mask over their slots (the internal children fill the
 // low slots, child slot c is node child_base + c).  One entry per tree level
@@ -149,6 +156,7 @@ struct TraceState {
     HitRec best;
     uint32_t sp;
     uint32_t next;  // node to visit next (the nearest hit child), or kNoNode
+    uint32_t pend;  // leaf entries queued, (first triangle << 3) | count each
 };
 
 LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float tmin, float tmax, bool has_nodes) {
@@ -163,6 +171,7 @@ LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float t
     s.best.u = s.best.v = 0.0f;
     s.rp = prep_ray(d);
     s.sp = 0;
+    s.pend = 0;
     s.next = has_nodes ? 0u : kNoNode;
     (void)stk;
 }
@@ -174,14 +183,15 @@ LT_DEV void trace_init(TraceState& s, const TraceStack& stk, f3 o, f3 d, float t
 template <bool COUNT>
 LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack& stk, bool any, uint32_t& n_nodes,
                            uint32_t& n_tris) {
-    if (s.next == kNoNode && s.sp == 0) return true;
+    const bool has_node = s.next != kNoNode || s.sp != 0;
+    if (!has_node && s.pend == 0) return true;
     const f3 o = s.o;
     const RayPre& rp = s.rp;
     const float tmin = s.tmin, tmax = s.tmax;
     float& tcur = s.tcur;
     HitRec& best = s.best;
     uint32_t& sp = s.sp;
-    {
+    if (has_node && s.pend <= (uint32_t)LT_NODE_PEND_MAX) {
         uint32_t ni = s.next;
         if (ni == kNoNode) {
             const uint32_t e = stk_top(stk, sp);
@@ -263,39 +273,15 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
         // hit slots that are not internal: leaves, and (rarely, within the
         // rounding margin) empty slots, whose count of 0 tests nothing
         uint32_t hit_leaf = hm & ~imask;
-        // leaves first: shrink tcur before ordering the internal children
+        // the hit leaves join the lane's queue; their triangles are tested
+        // below, a few lanes' worth at a time
         while (hit_leaf) {
             const int c = __builtin_ctz(hit_leaf);
             hit_leaf &= hit_leaf - 1;
             const uint32_t meta = ((c < 4 ? meta_lo : meta_hi) >> ((c & 3) * 8)) & 0xFF;
-            const uint32_t first = tri_base + (meta >> 3), cnt = meta & 7u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                const float4* tp = (const float4*)(sv.tris + first + k);
-                const float4 a = tp[0], b = tp[1], cc = tp[2];
-                if (COUNT) n_tris++;
-                float t, u, v;
-                if (isect_tri(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), o, rp, tmin, tmax, t, u,
-                              v)) {
-                    const uint32_t gid = __float_as_uint(cc.y);
-                    if (any) {
-                        best.t = t;
-                        best.u = u;
-                        best.v = v;
-                        best.gid = gid;
-                        best.prim = __float_as_uint(cc.z);
-                        best.inst = __float_as_uint(cc.w);
-                        return true;
-                    }
-                    if (t < best.t || (t == best.t && gid < best.gid)) {
-                        best.t = t;
-                        best.u = u;
-                        best.v = v;
-                        best.gid = gid;
-                        best.prim = __float_as_uint(cc.z);
-                        best.inst = __float_as_uint(cc.w);
-                        tcur = t;
-                    }
-                }
+            if (meta & 7u) {
+                stk.q[s.pend * kTraceBlock] = ((tri_base + (meta >> 3)) << 3) | (meta & 7u);
+                s.pend++;
             }
         }
         if (hit_int) {
@@ -316,7 +302,45 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
             s.next = child_base + (uint32_t)nearest;
         }
     }
-    return s.next == kNoNode && sp == 0;
+    // One queued triangle per lane and step: a node step and a triangle test
+    // share the step instead of the wave looping over the most leaf hits of
+    // any lane.  The other queued triangles wait for the next steps, which
+    // only delays the shrinking of tcur (culling stays conservative); an
+    // any-hit query ends at its first hit.
+    if (s.pend != 0u) {
+        const uint32_t e = stk.q[(s.pend - 1) * kTraceBlock];
+        const uint32_t cnt = e & 7u;
+        if (cnt == 1u)
+            s.pend--;
+        else
+            stk.q[(s.pend - 1) * kTraceBlock] = e - 1u;
+        const float4* tp = (const float4*)(sv.tris + (e >> 3) + cnt - 1u);
+        const float4 a = tp[0], b = tp[1], cc = tp[2];
+        if (COUNT) n_tris++;
+        float t, u, v;
+        if (isect_tri(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), o, rp, tmin, tmax, t, u, v)) {
+            const uint32_t gid = __float_as_uint(cc.y);
+            if (any) {
+                best.t = t;
+                best.u = u;
+                best.v = v;
+                best.gid = gid;
+                best.prim = __float_as_uint(cc.z);
+                best.inst = __float_as_uint(cc.w);
+                return true;
+            }
+            if (t < best.t || (t == best.t && gid < best.gid)) {
+                best.t = t;
+                best.u = u;
+                best.v = v;
+                best.gid = gid;
+                best.prim = __float_as_uint(cc.z);
+                best.inst = __float_as_uint(cc.w);
+                tcur = t;
+            }
+        }
+    }
+    return s.next == kNoNode && sp == 0 && s.pend == 0u;
 }
 
 template <bool ANY, bool COUNT>

@@ -235,7 +235,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_PATH_MEGA_WAVES))) void k_path_mega(SceneView sv, PathLaunch L, float4* __restrict__ accum,
                                                            uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
                                                            unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;
@@ -297,7 +297,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_PATH_MEGA_WAVES))) void k_path_regen(
     SceneView sv, PathLaunch L, float4* __restrict__ accum, uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
     unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t lane = threadIdx.x & 63u;

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                                                           uint32_t ext_depth, uint32_t with_ext, uint32_t sec_depth,
                                                           uint32_t with_sec, uint32_t* __restrict__ spill,
                                                           unsigned long long* __restrict__ counts) {
-    __shared__ uint32_t s_stack[ltb::kLdsStack * kTraceBlock];
+    __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
     const uint32_t n_ext = with_ext ? ctl->q_len[ext_depth] : 0u;
