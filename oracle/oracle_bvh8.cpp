@@ -1,10 +1,12 @@
 // oracle_bvh8.cpp — host traversal of the device-built BVH8 (TEST INFRASTRUCTURE).
 //
-// Walks the exported 128-byte nodes with the same child decode, slab test,
-// leaves-first order and nearest-child-last push as the device kernel, so it
-// reports the exact per-ray node and triangle counts of the device traversal
-// (the unit of the HBM roofline) and isolates build defects from traversal
-// defects.  Results must equal brute force (oracle_trace.cpp).
+// Walks the exported 128-byte nodes with the same child decode, slab test and
+// nearest-child-first order as the device kernel, testing a node's leaf
+// triangles at once (the device queues them and tests one per step, so its
+// closest-hit bound shrinks later): its per-ray node and triangle counts are
+// the device's lower bound, and its hits must equal the device's and brute
+// force (oracle_trace.cpp), which isolates build defects from traversal
+// defects.
 #include <vector>
 
 #include "oracle.h"
