@@ -69,7 +69,10 @@ LT_DEV bool isect_tri(f3 A0, f3 B0, f3 C0, f3 o, const RayPre& rp, float tmin, f
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    // the double-precision edge test is rare (an edge function exactly 0):
+    // a wave-uniform branch keeps it off the common path
+    const bool edge0 = U == 0.0f || V == 0.0f || W == 0.0f;
+    if (__ballot(edge0) != 0ull && edge0) {
         double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
         U = (float)(CxBy - CyBx);
         double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
