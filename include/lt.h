@@ -507,6 +507,27 @@ typedef struct lt_scene_integrator_cfg {
 } lt_scene_integrator_cfg;
 lt_status lt_scene_integrator(const lt_host_scene* s, lt_scene_integrator_cfg* out);
 
+/* ---- OpenEXR image files (host only, no device needed) ------------------- */
+
+/* Stored pixel type of lt_exr_save (the OpenEXR pixel type codes). */
+#define LT_EXR_HALF 1
+#define LT_EXR_FLOAT 2
+/* Writes the RGB of a width x height RGBA32F image (rows top to bottom) as a
+ * single-part scanline OpenEXR file: channels B, G, R, no compression, HALF
+ * (the reference's choice) or FLOAT.  Replaces ImageUtils::save_exr
+ * (src/Framework/ImageUtils.cpp:23-89). */
+lt_status lt_exr_save(const char* path, const float* rgba, uint32_t width, uint32_t height, int32_t pixel_type,
+                      char* err, size_t err_len);
+/* Reads a single-part scanline OpenEXR file (channels R, G, B, A in any subset,
+ * HALF / FLOAT / UINT, compression NONE, ZIPS or ZIP) into RGBA32F, rows top to
+ * bottom; missing colour channels read 0 and a missing alpha 1.  With
+ * rgba == NULL only the size is returned; otherwise capacity (floats) must be
+ * at least 4 * width * height.  Replaces ImageUtils::load_exr
+ * (src/Framework/ImageUtils.cpp:9-21), the ground-truth loader of the RMSE pass
+ * (src/RayTracer/RayTracer.cpp:199-242). */
+lt_status lt_exr_load(const char* path, float* rgba, size_t capacity, uint32_t* width, uint32_t* height, char* err,
+                      size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

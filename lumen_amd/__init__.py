@@ -18,7 +18,7 @@ from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRA
                    LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
 
 __all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "VCM", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
-           "lib", "make_rays"]
+           "lib", "make_rays", "save_exr", "load_exr"]
 
 _KINDS = {"cornell": LT_SCENE_CORNELL, "terrain": LT_SCENE_TERRAIN, "caustics": LT_SCENE_CAUSTICS,
           "diff_wall": LT_SCENE_DIFF_WALL}
@@ -420,3 +420,28 @@ def make_rays(origins: np.ndarray, dirs: np.ndarray, tmin=0.0, tmax=1e30) -> np.
     r["tmin"] = tmin
     r["tmax"] = tmax
     return r
+
+
+def save_exr(path, rgba: np.ndarray, half: bool = True) -> None:
+    """Writes the RGB of an (H, W, 4) float32 image (row 0 at the top) as an
+    OpenEXR file, channels B, G, R, stored as HALF (the reference's
+    ImageUtils::save_exr, src/Framework/ImageUtils.cpp:23-89) or FLOAT."""
+    img = np.ascontiguousarray(rgba, dtype=np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError(f"save_exr wants an (H, W, 4) image, got {img.shape}")
+    err = C.create_string_buffer(512)
+    st = lib().lt_exr_save(str(path).encode(), img.ctypes.data, img.shape[1], img.shape[0], 1 if half else 2, err, 512)
+    check(st, what=f"lt_exr_save({path}): {err.value.decode()}")
+
+
+def load_exr(path) -> np.ndarray:
+    """Reads an OpenEXR file (NONE / ZIPS / ZIP scanlines) as an (H, W, 4)
+    float32 image (ImageUtils::load_exr, src/Framework/ImageUtils.cpp:9-21)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(512)
+    st = lib().lt_exr_load(str(path).encode(), None, 0, C.byref(w), C.byref(h), err, 512)
+    check(st, what=f"lt_exr_load({path}): {err.value.decode()}")
+    img = np.empty((h.value, w.value, 4), np.float32)
+    st = lib().lt_exr_load(str(path).encode(), img.ctypes.data, img.size, C.byref(w), C.byref(h), err, 512)
+    check(st, what=f"lt_exr_load({path}): {err.value.decode()}")
+    return img

@@ -188,6 +188,9 @@ _PROTOS = {
     "lt_scene_camera": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(lt_camera)]),
     "lt_scene_integrator": (C.c_int, [P, C.POINTER(lt_scene_integrator_cfg)]),
     "lt_scene_set_camera": (C.c_int, [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "lt_exr_save": (C.c_int, [C.c_char_p, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_char_p, C.c_size_t]),
+    "lt_exr_load": (C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_char_p,
+                              C.c_size_t]),
 }
 EXPORTED = tuple(_PROTOS)
 

@@ -2,10 +2,13 @@
 // (src/RayTracer/RayTracer.cpp:383-466) without window or GUI.  Loads a scene,
 // creates the integrator through the C++ adapter, renders `spp` frames
 // (render + update per frame, as RayTracer::update does) and writes the
-// running-mean image as a PFM (little-endian RGB float, rows bottom to top).
+// running-mean image: an OpenEXR file when the name ends in .exr (HALF as
+// ImageUtils::save_exr writes it, src/Framework/ImageUtils.cpp:23-89, or FLOAT
+// with --exr-float), otherwise a PFM (little-endian RGB float, rows bottom to top).
 //
 //   lt_render_cli --scene cornell|terrain|caustics|diff_wall|<scene.json>
-//                 [--integrator path] [--width W] [--height H] [--spp N] [--out image.pfm]
+//                 [--integrator path] [--width W] [--height H] [--spp N]
+//                 [--out image.exr|image.pfm] [--exr-float]
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +30,7 @@ static bool write_pfm(const std::string& path, const OutputTex& t) {
 
 int main(int argc, char** argv) {
     std::string scene = "cornell", integrator = "path", out;
+    bool exr_float = false;
     uint32_t w = 1920, h = 1080, spp = 16;  // 1920x1080: src/main.cpp:14-15
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -37,6 +41,7 @@ int main(int argc, char** argv) {
         else if (a == "--height") h = (uint32_t)std::stoul(next());
         else if (a == "--spp") spp = (uint32_t)std::stoul(next());
         else if (a == "--out") out = next();
+        else if (a == "--exr-float") exr_float = true;
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -67,7 +72,14 @@ int main(int argc, char** argv) {
         printf("{\"scene\": \"%s\", \"width\": %u, \"height\": %u, \"spp\": %u, \"seconds\": %.6f, "
                "\"bvh_nodes\": %u, \"bvh_build_ms\": %.3f, \"mean_rgb\": %.6f}\n",
                scene.c_str(), w, h, spp, secs, bs.num_nodes, bs.build_ms, sum / (3.0 * w * h));
-        if (!out.empty() && !write_pfm(out, tex)) fprintf(stderr, "cannot write %s\n", out.c_str());
+        if (out.size() > 4 && out.compare(out.size() - 4, 4, ".exr") == 0) {
+            char err[512] = {0};
+            if (lt_exr_save(out.c_str(), tex.rgba.data(), tex.width, tex.height,
+                            exr_float ? LT_EXR_FLOAT : LT_EXR_HALF, err, sizeof(err)) != LT_OK)
+                fprintf(stderr, "%s\n", err);
+        } else if (!out.empty() && !write_pfm(out, tex)) {
+            fprintf(stderr, "cannot write %s\n", out.c_str());
+        }
         it->destroy();
         lt_context_destroy(ctx);
     } catch (const std::exception& e) {

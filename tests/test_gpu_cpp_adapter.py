@@ -40,3 +40,20 @@ def test_cli_matches_python_integrator(gpu_ctx):
     py = it.output_tex[..., :3]
     it.destroy()
     assert np.array_equal(cli, py)
+
+
+def test_cli_writes_exr(gpu_ctx):
+    """--out *.exr writes what ImageUtils::save_exr writes (HALF), --exr-float
+    the FLOAT file; both read back as the PFM image of the same run."""
+    print("[gpu] cli cornell 64x48 x2 exr", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        imgs = {}
+        for name, extra in (("a.pfm", []), ("a.exr", []), ("f.exr", ["--exr-float"])):
+            out = Path(d) / name
+            r = subprocess.run([str(CLI), "--scene", "cornell", "--width", "64", "--height", "48", "--spp", "2",
+                                "--out", str(out), *extra], capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            imgs[name] = read_pfm(out) if name.endswith(".pfm") else la.load_exr(out)[..., :3]
+    ref = imgs["a.pfm"]
+    assert np.array_equal(imgs["f.exr"], ref)
+    assert np.array_equal(imgs["a.exr"], ref.astype(np.float16).astype(np.float32))
