@@ -502,6 +502,9 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
 #ifndef LT_VCM_MERGE_WAVES
 #define LT_VCM_MERGE_WAVES 3
 #endif
+#ifndef LT_VCM_CELL_DIV
+#define LT_VCM_CELL_DIV 2
+#endif
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_MERGE_WAVES))) void k_vcm_merge(
     SceneView sv, VcmLaunch L, Grid Gr, const VcmQuery* __restrict__ queries, const uint32_t* __restrict__ n_queries,
     float4* __restrict__ fcol) {
@@ -946,6 +949,12 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
         vcm_frame_words(seed, L.frame_num, &L.light_word, &L.eye_word);
         vcm_radius_grid(L.bmin, L.bmax, cfg.radius_factor, cfg.alpha, L.frame_num, &L.radius, L.grid_res);
         for (int k = 0; k < 3; k++) L.inv_ext[k] = 1.0f / (L.bmax[k] - L.bmin[k]);
+        // The corrected estimator's cells are LT_VCM_CELL_DIV times finer per
+        // axis than the reference's (about r / LT_VCM_CELL_DIV wide): a query's
+        // box of +-r then covers (2 LT_VCM_CELL_DIV + 1)^3 smaller cells whose
+        // photons lie closer to the radius.  Its sums do not depend on the grid.
+        if (L.use_vm && L.corrected)
+            for (int k = 0; k < 3; k++) L.grid_res[k] *= LT_VCM_CELL_DIV;
         const float r2 = L.radius * L.radius;
         L.eta_vcm = PI * r2 * (float)N;
         L.eta_vc = 1.0f / L.eta_vcm;
