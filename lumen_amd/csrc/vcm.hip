@@ -509,6 +509,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     SceneView sv, VcmLaunch L, Grid Gr, const VcmQuery* __restrict__ queries, const uint32_t* __restrict__ n_queries,
     float4* __restrict__ fcol) {
     const uint32_t lane = threadIdx.x & 63u;
+    // photons of the wave's query that passed the distance and cell tests,
+    // waiting for a full wave of merges (at most 63 + 64 pending)
+    __shared__ uint32_t s_k[kTraceBlock / 64u][128];
+    __shared__ float s_d2[kTraceBlock / 64u][128];
+    uint32_t* const wk = s_k[threadIdx.x / 64u];
+    float* const wd2 = s_d2[threadIdx.x / 64u];
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t Q = *n_queries;
     const float r2 = L.radius * L.radius;
     const float norm = 1.0f / (PI * r2 * (float)L.n_paths);
@@ -534,6 +541,16 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
         const int ncells = (hi[0] - lo[0] + 1) * ny * nz;
         f3 part = mk3(0.0f);
+        uint32_t pend = 0u;  // wave-uniform
+        auto merge_slot = [&](uint32_t slot) {
+            const uint32_t k = wk[slot];
+            const VcmPhotonRest& pr = Gr.prest[k];
+            const float4 b = pr.b, c = pr.c, d = pr.d;
+            f3 r;
+            if (merge_one(L, wd2[slot], f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, Gr.ppos[k].w, b.w,
+                          __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
+                part += r;
+        };
         for (int cbase = 0; cbase < ncells; cbase += 64) {
             const int ci = cbase + (int)lane;  // this lane's cell of the chunk: its photon range
             uint32_t st = 0u, cnt = 0u;
@@ -562,24 +579,58 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                 j = j < 63 ? j : 63;
                 const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
                 const int jx = __shfl(cx, j), jy = __shfl(cy, j), jz = __shfl(cz, j);
-                if (t >= total) continue;
-                const uint32_t k = sj + (t - ej);
-                const float4 a = Gr.ppos[k];
-                const f3 ppos = {a.x, a.y, a.z};
-                const f3 pp = qpos - ppos;
-                const float dist_sqr = dot(pp, pp);
-                if (dist_sqr > r2) continue;
-                int g[3];
-                grid_idx_fast(L, ppos, g);
-                if (g[0] != jx || g[1] != jy || g[2] != jz) continue;  // another cell of the bucket
-                const VcmPhotonRest& pr = Gr.prest[k];
-                const float4 b = pr.b, c = pr.c, d = pr.d;
-                f3 r;
-                if (merge_one(L, dist_sqr, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, a.w, b.w,
-                              __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
-                    part += r;
+                bool keep = false;
+                uint32_t k = 0u;
+                float dist_sqr = 0.0f;
+                if (t < total) {
+                    k = sj + (t - ej);
+                    const float4 a = Gr.ppos[k];
+                    const f3 ppos = {a.x, a.y, a.z};
+                    const f3 pp = qpos - ppos;
+                    dist_sqr = dot(pp, pp);
+                    if (dist_sqr <= r2) {
+                        int g[3];
+                        grid_idx_fast(L, ppos, g);
+                        keep = g[0] == jx && g[1] == jy && g[2] == jz;  // else another cell of the bucket
+                    }
+                }
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t slot = pend + (uint32_t)__popcll(m & below);
+                    wk[slot] = k;
+                    wd2[slot] = dist_sqr;
+                }
+                pend += (uint32_t)__popcll(m);
+                if (pend >= 64u) {  // one full wave of merges, then the rest moves down
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    merge_slot(lane);
+                    pend -= 64u;
+                    uint32_t mk = 0u;
+                    float md = 0.0f;
+                    if (lane < pend) {
+                        mk = wk[64u + lane];
+                        md = wd2[64u + lane];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane < pend) {
+                        wk[lane] = mk;
+                        wd2[lane] = md;
+                    }
+                }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < pend) merge_slot(lane);
+        // the next query's writes must follow every lane's reads of this one
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int o = 32; o > 0; o >>= 1) {
             part.x += __shfl_xor(part.x, o);
             part.y += __shfl_xor(part.y, o);
