@@ -1,5 +1,6 @@
 """GPU BVH8 traversal vs the oracle: bit-exact closest-hit (t, inst, prim, u, v)
-and any-hit on seeded ray sets (the reference's own check)."""
+and any-hit on seeded ray sets, brute force over every triangle being the
+ground truth."""
 import numpy as np
 import pytest
 
