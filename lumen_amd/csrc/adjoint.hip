@@ -30,6 +30,10 @@
 #include "adjoint.h"
 #include "context.h"
 #include "dev_scene.h"
+// The adjoint kernels test each hit leaf before taking another node: their
+// traversals run beside a long per-lane replay state, and the leaf queue's
+// deferred culling cost them more node steps than it saved in idle lanes.
+#define LT_NODE_PEND_MAX 0
 #include "dev_trace.h"
 
 using namespace ltd;
