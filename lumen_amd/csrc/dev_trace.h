@@ -142,6 +142,9 @@ LT_DEV void stk_set_top(const TraceStack& s, uint32_t sp, uint32_t v) {
 #ifndef LT_NODE_PEND_MAX
 #define LT_NODE_PEND_MAX (ltb::kLeafQueue - 8)
 #endif
+// a node step queues up to 8 more entries: the queue must hold them
+static_assert(LT_NODE_PEND_MAX >= 0 && LT_NODE_PEND_MAX + 8 <= (int)ltb::kLeafQueue,
+              "LT_NODE_PEND_MAX + 8 leaf entries exceed the per-lane LDS leaf queue");
 
 // A stack entry is a group of sibling internal nodes still to visit:
 // (child_base << 8) | mask over their slots (the internal children fill the
