@@ -338,6 +338,71 @@ class Accel:
             pass
 
 
+class AdjParams64(C.Structure):
+    """ora_adj_params of the float64 build (its sky_col holds doubles)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32), ("sky_col", C.c_double * 3), ("albedo_id", C.c_int32), ("rough_id", C.c_int32),
+                ("aw", C.c_uint32), ("ah", C.c_uint32), ("rw", C.c_uint32), ("rh", C.c_uint32)]
+
+
+_lib64 = None
+
+
+def lib64() -> C.CDLL:
+    """The differentiable estimator compiled in float64 (oracle_f64.cpp)."""
+    global _lib64
+    if _lib64 is None:
+        path = HERE / "_build" / "liboracle_f64.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        P = C.c_void_p
+        L.ora64_accel_build.argtypes = [P]
+        L.ora64_accel_build.restype = P
+        L.ora64_accel_free.argtypes = [P]
+        L.ora_adjoint.argtypes = [P, P, P, C.POINTER(AdjParams64), C.c_uint32, P, P, P, P, P, P, P, P, C.c_int]
+        L.ora_adjoint.restype = C.c_double
+        _lib64 = L
+    return _lib64
+
+
+class Accel64:
+    """The forward of Accel.adjoint evaluated in float64: same scene, camera,
+    seed, random numbers and sampling decisions (up to ties within a rounding
+    error), so central differences of it carry float64 rounding only."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self._h = lib64().ora64_accel_build(C.byref(desc))
+
+    def adjoint(self, camera, width, height, max_depth, spp, albedo=None, rough=None, albedo_id=-1, rough_id=-1,
+                seed=0, sky_col=(0, 0, 0), eval_albedo=None, eval_rough=None, threads=0):
+        """images float64 [V, H, W, 3] of the texel variants (as Accel.adjoint's forward)."""
+        V = (albedo if albedo is not None else rough).shape[0]
+        p = AdjParams64()
+        p.width, p.height, p.max_depth, p.spp, p.seed = width, height, max_depth, spp, seed
+        p.sky_col[:] = list(sky_col)
+        p.albedo_id, p.rough_id = albedo_id, rough_id
+        a = np.ascontiguousarray(albedo if albedo is not None else np.zeros((V, 1, 1, 3)), np.float64)
+        r = np.ascontiguousarray(rough if rough is not None else np.zeros((V, 1, 1)), np.float64)
+        p.ah, p.aw = a.shape[1:3]
+        p.rh, p.rw = r.shape[1:3]
+        ea = None if eval_albedo is None else np.ascontiguousarray(eval_albedo, np.float64)
+        er = None if eval_rough is None else np.ascontiguousarray(eval_rough, np.float64)
+        img = np.zeros((V, height, width, 3), np.float64)
+        lib64().ora_adjoint(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), V, a.ctypes.data,
+                            r.ctypes.data, None if ea is None else ea.ctypes.data,
+                            None if er is None else er.ctypes.data, img.ctypes.data, None, None, None,
+                            threads or lib().ora_hardware_threads())
+        return img
+
+    def __del__(self):
+        try:
+            lib64().ora64_accel_free(self._h)
+        except Exception:
+            pass
+
+
 def bvh8_trace(nodes: np.ndarray, tris: np.ndarray, rays: np.ndarray, any_hit: bool = False):
     """Host traversal of an exported device BVH8: (hits or occluded, node counts, tri counts)."""
     from lumen_amd._abi import HIT_DTYPE, RAY_DTYPE  # record layouts only

@@ -38,6 +38,11 @@
 
 namespace ora {
 
+// A genuine 32-bit float, also in the float64 build of oracle_f64.cpp (which
+// widens the keyword `float`; a float literal keeps its type there): the
+// bit-level steps (uint_to_float, the integer ulp push of offset_ray) use it.
+typedef __typeof__(1.0f) f32;
+
 constexpr float PI = 3.14159265359f;       // utils.glsl:4
 constexpr float TWO_PI = 6.28318530718f;   // utils.glsl:5
 constexpr float INV_PI = 1.0f / PI;        // utils.glsl:6 (float division)
@@ -88,9 +93,13 @@ inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 inline float luminance(vec3 c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }  // utils.glsl:99
 inline vec3 load3(const float* p) { return {p[0], p[1], p[2]}; }
+#ifdef ORA_F64
+inline vec3 load3(const f32* p) { return {p[0], p[1], p[2]}; }
+#endif
 
 // Column-major 4x4 (glm): m[c*4 + r].
-inline vec4 mul(const float* m, vec4 v) {
+template <class R>  // R = the matrix's element type (float32 in the wire format)
+inline vec4 mul(const R* m, vec4 v) {
     vec4 r;
     r.x = m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * v.w;
     r.y = m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * v.w;
@@ -157,6 +166,32 @@ struct Material {
         sheen, subsurface, flatness, anisotropy;
     uint32_t thin;
 };
+// Field-wise copy of the wire-format material (so that the float64 build,
+// whose Material holds doubles, converts instead of reinterpreting bytes).
+inline Material material_from(const lt_material& w) {
+    Material m;
+    m.albedo = load3(w.albedo);
+    m.ior = w.ior;
+    m.emissive_factor = load3(w.emissive_factor);
+    m.bsdf_type = w.bsdf_type;
+    m.bsdf_props = w.bsdf_props;
+    m.k = load3(w.k);
+    m.texture_id = w.texture_id;
+    m.roughness = w.roughness;
+    m.diffuse_trans = w.diffuse_trans;
+    m.spec_trans = w.spec_trans;
+    m.metallic = w.metallic;
+    m.specular_tint = w.specular_tint;
+    m.sheen_tint = w.sheen_tint;
+    m.clearcoat = w.clearcoat;
+    m.clearcoat_gloss = w.clearcoat_gloss;
+    m.sheen = w.sheen;
+    m.subsurface = w.subsurface;
+    m.flatness = w.flatness;
+    m.anisotropy = w.anisotropy;
+    m.thin = w.thin;
+    return m;
+}
 Material load_material(const lt_scene_desc& s, uint32_t idx, vec2 uv);
 vec3 sample_bsdf(vec3 n_s, vec3 wo, const Material& m, uint32_t mode, bool side, vec3& wi, float& pdf_w,
                  float& cos_theta, vec3 rands);
@@ -180,7 +215,7 @@ struct TriangleRecord {
     float triangle_pdf;
 };
 TriangleRecord sample_triangle(const lt_scene_desc& s, const lt_prim_mesh_info& pi, vec2 rands, uint32_t tri,
-                               const float* world, vec2& uv);
+                               const f32* world, vec2& uv);
 vec3 sample_light_Li(const lt_scene_desc& s, vec4 rands_pos, vec3 p, int num_lights, float& pdf_pos_w, vec3& wi,
                      float& wi_len, float& pdf_pos_a, float& cos_from_light, LightRecord& rec, vec3& n, vec3& pos,
                      float& pdf_pos_dir_w);

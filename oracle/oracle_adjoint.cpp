@@ -511,7 +511,7 @@ double ora_adjoint(const lt_scene_desc* s, void* accel, const lt_camera* cam, co
 void ora_bsdf_eval_d(const lt_material* m, const float* n_s, const float* wo, const float* wi, int side,
                      float* out) {
     DMat d;
-    memcpy(&d.m, m, sizeof(Material));
+    d.m = material_from(*m);
     d.tex_albedo = d.tex_rough = true;
     d.albedo_mult = v3(1.0f);
     float pdf;

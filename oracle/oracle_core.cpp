@@ -29,7 +29,7 @@ uvec4 pcg4d(uvec4 v) {
 // utils.glsl:136
 float uint_to_float(uint32_t x) {
     uint32_t b = 0x3f800000u | (x >> 9);
-    float f;
+    f32 f;
     memcpy(&f, &b, 4);
     return f - 1.0f;
 }
@@ -41,8 +41,8 @@ float rand(uvec4& s) {
 }
 
 // ------------------------------------------------------------ geometry ----
-static inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
-static inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
+static inline int f2i(f32 f) { int i; memcpy(&i, &f, 4); return i; }
+static inline f32 i2f(int i) { f32 f; memcpy(&f, &i, 4); return f; }
 
 // utils.glsl:73-84 (Wächter & Binder, Ray Tracing Gems ch. 6)
 vec3 offset_ray(vec3 p, vec3 n) {
@@ -784,10 +784,10 @@ static vec4 tex_bilinear(const lt_texture& t, vec2 uv) {
     int w = (int)t.width, h = (int)t.height;
     int x0 = ((int)x0f % w + w) % w, y0 = ((int)y0f % h + h) % h;
     int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
-    const float* a = t.data + 4 * ((size_t)y0 * w + x0);
-    const float* b = t.data + 4 * ((size_t)y0 * w + x1);
-    const float* c = t.data + 4 * ((size_t)y1 * w + x0);
-    const float* d = t.data + 4 * ((size_t)y1 * w + x1);
+    const f32* a = t.data + 4 * ((size_t)y0 * w + x0);
+    const f32* b = t.data + 4 * ((size_t)y0 * w + x1);
+    const f32* c = t.data + 4 * ((size_t)y1 * w + x0);
+    const f32* d = t.data + 4 * ((size_t)y1 * w + x1);
     vec4 r;
     float* rp = &r.x;
     for (int i = 0; i < 4; i++) {
@@ -800,9 +800,7 @@ static vec4 tex_bilinear(const lt_texture& t, vec2 uv) {
 
 // bsdf_commons.glsl:16-22 (+ per-material roughness texture, an addition)
 Material load_material(const lt_scene_desc& s, uint32_t idx, vec2 uv) {
-    Material m;
-    static_assert(sizeof(Material) == sizeof(lt_material), "layout");
-    memcpy(&m, &s.materials[idx], sizeof(Material));
+    Material m = material_from(s.materials[idx]);
     if (m.texture_id > -1) {
         vec4 t = tex_bilinear(s.textures[m.texture_id], uv);
         m.albedo = m.albedo * v3(t.x, t.y, t.z);
@@ -882,7 +880,7 @@ static inline uint32_t light_type(uint32_t flags) { return flags & 7u; }
 
 // commons.glsl:112-149
 TriangleRecord sample_triangle(const lt_scene_desc& s, const lt_prim_mesh_info& pi, vec2 rands, uint32_t tri,
-                               const float* world, vec2& uv) {
+                               const f32* world, vec2& uv) {
     TriangleRecord r;
     uint32_t io = pi.index_offset + 3 * tri;
     uint32_t vo = pi.vertex_offset;
