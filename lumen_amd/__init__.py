@@ -216,6 +216,14 @@ class Context:
               "lt_image_sq_error")
         return out.value
 
+    def rmse(self, a: np.ndarray, b: np.ndarray):
+        """(Lumen's RMSE, standard RMSE) of two (H, W, 4) images over rgb:
+        sqrt(S) / (3 N) as output_rmse.comp:23 computes it, and sqrt(S / (3 N)),
+        with S = sum over pixels of |a.rgb - b.rgb|^2 reduced on the GPU."""
+        S = self.sq_error(a, b)
+        n = a.size // 4
+        return (float(np.sqrt(S)) / (3 * n), float(np.sqrt(S / (3 * n)))) if n else (0.0, 0.0)
+
     def close(self):
         if self._h:
             lib().lt_context_destroy(self._h)

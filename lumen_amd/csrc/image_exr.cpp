@@ -27,6 +27,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -169,7 +170,7 @@ void unfilter(std::vector<uint8_t>& buf, std::vector<uint8_t>& tmp) {
 
 }  // namespace
 
-extern "C" lt_status lt_exr_save(const char* path, const float* rgba, uint32_t width, uint32_t height,
+static lt_status exr_save_impl(const char* path, const float* rgba, uint32_t width, uint32_t height,
                                  int32_t pixel_type, char* err, size_t err_len) {
     if (!path || !rgba || width == 0 || height == 0 || width > (1u << 30) / height)
         return fail(err, err_len, LT_ERR_INVALID_ARG, "lt_exr_save: null pointer or bad size");
@@ -234,8 +235,22 @@ extern "C" lt_status lt_exr_save(const char* path, const float* rgba, uint32_t w
     return LT_OK;
 }
 
-extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity, uint32_t* width_out,
-                                 uint32_t* height_out, char* err, size_t err_len) {
+extern "C" lt_status lt_exr_save(const char* path, const float* rgba, uint32_t width, uint32_t height,
+                                 int32_t pixel_type, char* err, size_t err_len) {
+    try {
+        return exr_save_impl(path, rgba, width, height, pixel_type, err, err_len);
+    } catch (const std::bad_alloc&) {
+        return fail(err, err_len, LT_ERR_OUT_OF_MEMORY, "lt_exr_save: out of host memory");
+    }
+}
+
+// Limits on what a file may ask of the reader: channels per file and bytes of
+// one decoded chunk (ZIP: 16 scanlines); beyond them the file is rejected.
+constexpr size_t kMaxChannels = 64;
+constexpr uint64_t kMaxChunkBytes = 1ull << 31;
+
+static lt_status exr_load_impl(const char* path, float* rgba, size_t capacity, uint32_t* width_out,
+                               uint32_t* height_out, char* err, size_t err_len) {
     if (!path || !width_out || !height_out)
         return fail(err, err_len, LT_ERR_INVALID_ARG, "lt_exr_load: null pointer");
     FILE* fp = fopen(path, "rb");
@@ -273,6 +288,8 @@ extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity,
                     return fail(err, err_len, LT_ERR_PARSE, "%s: channel %s has pixel type %d", path,
                                 c.name.c_str(), c.type);
                 c.slot = c.name == "R" ? 0 : c.name == "G" ? 1 : c.name == "B" ? 2 : c.name == "A" ? 3 : -1;
+                if (chans.size() == kMaxChannels)
+                    return fail(err, err_len, LT_ERR_UNSUPPORTED, "%s: more than %zu channels", path, kMaxChannels);
                 chans.push_back(c);
             }
             if (!a.ok) return fail(err, err_len, LT_ERR_PARSE, "%s: truncated channel list", path);
@@ -302,9 +319,13 @@ extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity,
     uint64_t line_bytes = 0;
     for (const Channel& c : chans) line_bytes += (uint64_t)type_bytes(c.type) * W;
     const uint32_t lines_per_chunk = comp == kZip ? 16u : 1u;
+    if (line_bytes * lines_per_chunk > kMaxChunkBytes)
+        return fail(err, err_len, LT_ERR_UNSUPPORTED, "%s: a chunk of %llu bytes", path,
+                    (unsigned long long)(line_bytes * lines_per_chunk));
     const uint32_t nchunks = (H + lines_per_chunk - 1) / lines_per_chunk;
     for (size_t i = 0; i < 4ull * W * H; i++) rgba[i] = (i & 3) == 3 ? 1.f : 0.f;
     std::vector<uint8_t> buf, tmp;
+    std::vector<uint8_t> seen(H, 0);  // every scanline exactly once
     for (uint32_t k = 0; k < nchunks; k++) {
         Reader t{file.data(), file.size(), r.at + 8ull * k};
         const uint64_t off = t.u64();
@@ -316,6 +337,10 @@ extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity,
             return fail(err, err_len, LT_ERR_PARSE, "%s: bad chunk %u", path, k);
         const uint32_t row0 = (uint32_t)(y0 - box[1]);
         const uint32_t nl = std::min(lines_per_chunk, H - row0);
+        for (uint32_t l = 0; l < nl; l++) {
+            if (seen[row0 + l]) return fail(err, err_len, LT_ERR_PARSE, "%s: scanline %u given twice", path, row0 + l);
+            seen[row0 + l] = 1;
+        }
         const uint64_t expect = line_bytes * nl;
         const uint8_t* src = c.p + c.at;
         if (bytes != expect) {  // compressed chunk (an incompressible one is stored raw)
@@ -351,5 +376,18 @@ extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity,
             }
         }
     }
+    for (uint32_t y = 0; y < H; y++)
+        if (!seen[y]) return fail(err, err_len, LT_ERR_PARSE, "%s: scanline %u missing", path, y);
     return LT_OK;
+}
+
+extern "C" lt_status lt_exr_load(const char* path, float* rgba, size_t capacity, uint32_t* width_out,
+                                 uint32_t* height_out, char* err, size_t err_len) {
+    try {
+        return exr_load_impl(path, rgba, capacity, width_out, height_out, err, err_len);
+    } catch (const std::bad_alloc&) {
+        return fail(err, err_len, LT_ERR_OUT_OF_MEMORY, "%s: out of host memory", path ? path : "lt_exr_load");
+    } catch (...) {
+        return fail(err, err_len, LT_ERR_PARSE, "%s: unreadable file", path ? path : "lt_exr_load");
+    }
 }

@@ -147,12 +147,110 @@ class Path final : public Integrator {
     lt_camera camera_{};
 };
 
+// Bidirectional path tracing (src/RayTracer/BDPT.{h,cpp}): subpaths of up to
+// path_length + 1 vertices (BDPT.cpp:8-18), MIS with the reference's balance
+// heuristic (calc_mis_weight, bdpt_commons.glsl:272-473) or the power heuristic.
+class BDPT final : public Integrator {
+   public:
+    BDPT(LumenScene* scene, lt_context* ctx, uint32_t w, uint32_t h, uint32_t mis = LT_MIS_BALANCE)
+        : Integrator(scene, ctx, w, h), mis_(mis) {}
+    void init() override {
+        check(lt_scene_upload(ctx_, &lumen_scene->desc()), ctx_, "lt_scene_upload");
+        lt_scene_integrator_cfg sc = lumen_scene->config();
+        lt_bdpt_config cfg{};
+        cfg.width = w_;
+        cfg.height = h_;
+        cfg.max_depth = sc.path_length > 0 ? sc.path_length : 6;  // SceneConfig.h:16
+        cfg.mis = mis_;
+        check(lt_integrator_create(ctx_, LT_INTEGRATOR_BDPT, &cfg, sizeof(cfg), &it_), ctx_, "lt_integrator_create");
+        camera_ = lumen_scene->camera(w_, h_);
+        frame_num = 0;  // BDPT.cpp:42
+    }
+    // BDPT::render (BDPT.cpp:55-91): the reference reseeds each frame with
+    // rand() (pc_ray.time); here the RNG frame word is frame_num ^ seed, so a
+    // run is reproducible.
+    void render() override {
+        lt_frame_params p{};
+        p.camera = camera_;
+        p.frame_num = frame_num;
+        p.seed = seed;
+        p.spp = 1;
+        check(lt_render(it_, &p), ctx_, "lt_render");
+    }
+
+   private:
+    uint32_t mis_;
+    lt_camera camera_{};
+};
+
+// Vertex connection and merging (src/RayTracer/VCM.{h,cpp}): the scene's
+// VCMConfig (enable_vm, radius_factor; SceneConfig.h:38-42), the merge radius
+// r_0 (n + 1)^(-(1 - alpha) / 2) with the reference's alpha = 2/3 by default
+// (VCM.cpp:140-141), and VCM::update's frame counter (VCM.cpp:273-280).
+class VCM final : public Integrator {
+   public:
+    VCM(LumenScene* scene, lt_context* ctx, uint32_t w, uint32_t h, uint32_t estimator = LT_VCM_REFERENCE,
+        float alpha = 2.0f / 3.0f)
+        : Integrator(scene, ctx, w, h), estimator_(estimator), alpha_(alpha) {}
+    void init() override {
+        check(lt_scene_upload(ctx_, &lumen_scene->desc()), ctx_, "lt_scene_upload");
+        lt_scene_integrator_cfg sc = lumen_scene->config();
+        lt_vcm_config cfg{};
+        cfg.width = w_;
+        cfg.height = h_;
+        cfg.max_depth = sc.path_length > 0 ? sc.path_length : 6;
+        for (int k = 0; k < 3; k++) cfg.sky_col[k] = sc.sky_col[k];
+        cfg.radius_factor = sc.radius_factor > 0.0f ? sc.radius_factor : 0.025f;  // SceneConfig.h:39
+        cfg.alpha = alpha_;
+        cfg.enable_vm = sc.enable_vm ? 1u : 0u;
+        cfg.estimator = estimator_;
+        check(lt_integrator_create(ctx_, LT_INTEGRATOR_VCM, &cfg, sizeof(cfg), &it_), ctx_, "lt_integrator_create");
+        camera_ = lumen_scene->camera(w_, h_);
+        frame_num = 0;  // VCM.cpp:105
+    }
+    void render() override {
+        lt_frame_params p{};
+        p.camera = camera_;
+        p.frame_num = frame_num;
+        p.seed = seed;
+        p.spp = 1;
+        check(lt_render(it_, &p), ctx_, "lt_render");
+    }
+
+   private:
+    uint32_t estimator_;
+    float alpha_;
+    lt_camera camera_{};
+};
+
+// Options of the factory beyond the scene's own integrator block.
+struct IntegratorOptions {
+    uint32_t path_strategy = LT_PATH_MEGAKERNEL;  // Path execution form
+    uint32_t mis = LT_MIS_BALANCE;                // BDPT heuristic
+    uint32_t vcm_estimator = LT_VCM_REFERENCE;    // VCM estimator
+    float vcm_alpha = 2.0f / 3.0f;                // VCM radius reduction (VCM.cpp:141)
+};
+
 // RayTracer::create_integrator, keyed by the IntegratorType names
 // (src/RayTracer/SceneConfig.h:13) that this core builds.
 inline std::unique_ptr<Integrator> create_integrator(const std::string& name, LumenScene* scene, lt_context* ctx,
-                                                     uint32_t w, uint32_t h) {
-    if (name == "path" || name == "Path") return std::unique_ptr<Integrator>(new Path(scene, ctx, w, h));
+                                                     uint32_t w, uint32_t h,
+                                                     const IntegratorOptions& o = IntegratorOptions()) {
+    if (name == "path" || name == "Path")
+        return std::unique_ptr<Integrator>(new Path(scene, ctx, w, h, o.path_strategy));
+    if (name == "bdpt" || name == "BDPT") return std::unique_ptr<Integrator>(new BDPT(scene, ctx, w, h, o.mis));
+    if (name == "vcm" || name == "VCM")
+        return std::unique_ptr<Integrator>(new VCM(scene, ctx, w, h, o.vcm_estimator, o.vcm_alpha));
     throw std::runtime_error("integrator '" + name + "' is not built");
+}
+
+// The IntegratorType name of the scene's JSON integrator block (lt_integrator_kind).
+inline std::string integrator_name(lt_integrator_kind k) {
+    switch (k) {
+        case LT_INTEGRATOR_BDPT: return "bdpt";
+        case LT_INTEGRATOR_VCM: return "vcm";
+        default: return "path";
+    }
 }
 
 }  // namespace lumen_amd

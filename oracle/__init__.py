@@ -190,6 +190,14 @@ def sample_light_Le(desc, rands6):
     return out
 
 
+def vcm_merge_stats():
+    """(queries, queries with > 64 photons in the radius, largest count) of
+    the last corrected-estimator render_vcm."""
+    out = (C.c_uint64 * 3)()
+    lib().ora_vcm_merge_stats(out)
+    return tuple(out)
+
+
 def vcm_words(seed: int, frame_num: int):
     lw, ew = C.c_uint32(), C.c_uint32()
     lib().ora_vcm_words(seed, frame_num, C.byref(lw), C.byref(ew))

@@ -359,10 +359,16 @@ static bool merge_one(const VcmCtx& c, const Photon& ph, int depth, vec3 n_s, ve
     return true;
 }
 
+// Merge-query statistics of the corrected estimator (for tests): queries,
+// queries with more than 64 photons inside the radius, and the largest such
+// count, over the frames of the last ora_render_vcm call.
+static std::atomic<uint64_t> g_mq_total{0}, g_mq_over64{0}, g_mq_max{0};
+
 // vcm_commons.glsl:251-296
 static vec3 merge(const VcmCtx& c, int depth, vec3 n_s, vec3 wo, const Material& mat, bool side, const VState& cs,
                   const Payload& pl, float norm) {
     vec3 res = v3(0.0f);
+    uint64_t in_radius = 0;
     int lo[3], hi[3];
     grid_idx(c, pl.pos - v3(c.radius), lo);
     grid_idx(c, pl.pos + v3(c.radius), hi);
@@ -381,12 +387,21 @@ static vec3 merge(const VcmCtx& c, int depth, vec3 n_s, vec3 wo, const Material&
                         int g[3];
                         grid_idx(c, v.pos, g);
                         if (g[0] != x || g[1] != y || g[2] != z) continue;  // another cell of the bucket
+                        const vec3 dp = pl.pos - v.pos;
+                        if (dot(dp, dp) <= c.radius * c.radius) in_radius++;
                         const Photon ph{v.pos, v.d_vm, -v.wi, v.d_vcm, v.thr, 1, v.n_s, v.path_len};
                         vec3 r;
                         if (merge_one(c, ph, depth, n_s, wo, mat, side, cs, pl, norm, r)) res += r;
                     }
                 }
             }
+    if (c.corrected) {
+        g_mq_total++;
+        if (in_radius > 64) g_mq_over64++;
+        uint64_t m = g_mq_max.load();
+        while (in_radius > m && !g_mq_max.compare_exchange_weak(m, in_radius)) {
+        }
+    }
     return res;
 }
 
@@ -536,6 +551,15 @@ typedef struct ora_vcm_params {
     float sky_col[3];
 } ora_vcm_params;
 
+// Merge statistics of the last ora_render_vcm call (corrected estimator):
+// out = [merge queries, queries with > 64 photons within the radius, the
+// largest number of photons within the radius of one query].
+void ora_vcm_merge_stats(uint64_t* out) {
+    out[0] = g_mq_total.load();
+    out[1] = g_mq_over64.load();
+    out[2] = g_mq_max.load();
+}
+
 // Per-frame RNG words: the light pass uses `random_num` (vcm_light.rgen:57-58)
 // and the eye pass `frame_num ^ random_num` (vcm_eye.rgen:35-36); random_num
 // (host rand(), VCM.cpp:147) is a pcg4d hash of (seed, frame) here.
@@ -579,6 +603,9 @@ double ora_render_vcm(const lt_scene_desc* s, void* accel, const lt_camera* cam,
     }
     c.verts.assign((size_t)N * (c.D + 1), VV{});
     c.cnt.assign(N, 0u);
+    g_mq_total = 0;
+    g_mq_over64 = 0;
+    g_mq_max = 0;
     if (threads < 1) threads = 1;
     std::atomic<uint64_t> cl{0}, sh{0};
     auto t0 = std::chrono::steady_clock::now();
@@ -686,6 +713,9 @@ int ora_vcm_grid_selfcheck(const lt_scene_desc* s, void* accel, const lt_camera*
     c.cam_area = fabsf(ai.x * ai.y);
     c.verts.assign((size_t)N * (c.D + 1), VV{});
     c.cnt.assign(N, 0u);
+    g_mq_total = 0;
+    g_mq_over64 = 0;
+    g_mq_max = 0;
     uint32_t lw, ew;
     ora_vcm_words(p->seed, p->frame_start, &lw, &ew);
     ora_vcm_radius_grid(s, p->radius_factor, p->alpha, p->frame_start, &c.radius, c.grid_res);

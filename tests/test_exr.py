@@ -224,3 +224,54 @@ def test_error_paths(tmp_path):
         la.save_exr(tmp_path / "no_such_dir" / "x.exr", img)
     assert lib.lt_exr_save(str(tmp_path / "x.exr").encode(), img.ctypes.data, 5, 4, 3, err, 256) == 1
     assert lib.lt_exr_save(str(tmp_path / "x.exr").encode(), None, 5, 4, 1, err, 256) == 1
+
+
+def _load_status(path):
+    w, h = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(256)
+    buf = np.zeros(4 * 64 * 64, np.float32)
+    st = la.lib().lt_exr_load(str(path).encode(), buf.ctypes.data, buf.size, C.byref(w), C.byref(h), err, 256)
+    return st, err.value.decode()
+
+
+def test_scanline_coverage_is_checked(tmp_path):
+    """Every scanline must come from exactly one chunk: a file whose offset
+    table points twice at the same chunk (one scanline given twice, another
+    missing) is rejected."""
+    img = np.random.default_rng(0).random((4, 5, 4)).astype(np.float32)
+    la.save_exr(tmp_path / "s.exr", img, half=False)
+    b = bytearray((tmp_path / "s.exr").read_bytes())
+    _, end = parse_header(bytes(b))
+    offs = list(struct.unpack("<4Q", b[end:end + 32]))
+    offs[2] = offs[1]  # chunk of scanline 1 twice, scanline 2 never
+    b[end:end + 32] = struct.pack("<4Q", *offs)
+    (tmp_path / "dup.exr").write_bytes(bytes(b))
+    st, msg = _load_status(tmp_path / "dup.exr")
+    assert st == 5 and "given twice" in msg
+    # a chunk header naming another scanline than its slot leaves one uncovered
+    b = bytearray((tmp_path / "s.exr").read_bytes())
+    o3 = offs[3]
+    b[o3:o3 + 4] = struct.pack("<i", 0)
+    (tmp_path / "gap.exr").write_bytes(bytes(b))
+    st, msg = _load_status(tmp_path / "gap.exr")
+    assert st == 5 and "given twice" in msg
+
+
+def test_channel_count_is_limited(tmp_path):
+    """A channel list longer than 64 entries is refused before any pixel
+    buffer is sized from it."""
+    img = np.ones((2, 2, 4), np.float32)
+    la.save_exr(tmp_path / "s.exr", img, half=False)
+    b = (tmp_path / "s.exr").read_bytes()
+    attrs, end = parse_header(b)
+    entry = lambda n: n.encode() + b"\0" + struct.pack("<iIii", 2, 0, 1, 1)
+    chl = b"".join(entry(f"X{i}") for i in range(65)) + b"\0"
+    hdr = struct.pack("<II", 20000630, 2)
+    for name, (typ, val) in attrs.items():
+        if name == "channels":
+            val = chl
+        hdr += name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(val)) + val
+    hdr += b"\0"
+    (tmp_path / "many.exr").write_bytes(hdr + b[end:])
+    st, msg = _load_status(tmp_path / "many.exr")
+    assert st == 6 and "channels" in msg

@@ -60,3 +60,16 @@ def test_vcm_sealed_cornell(gpu_ctx, estimator, vm):
     sc = _DescScene(sealed_cornell(), camera([0, 0, 3.9], [0, 0, -1], 45), 5)
     g, o = _render_both(gpu_ctx, sc, 32, 32, 3, 5, estimator, vm, 1.0, seed=11)
     _compare(g, o, f"vcm sealed {estimator} vm={vm}")
+
+
+def test_vcm_dense_merge_queries(gpu_ctx):
+    """A small image with a large merge radius over the caustic: many merge
+    queries hold more than 64 photons inside the radius, so the merge
+    kernel's LDS queue fills and carries over (corrected estimator)."""
+    print("[gpu] vcm dense merge queries", flush=True)
+    sc = la.Scene.procedural("caustics", seed=7)
+    g, o = _render_both(gpu_ctx, sc, 64, 64, 2, 8, "corrected", True, 4.0, seed=5)
+    q, over, mx = oracle.vcm_merge_stats()
+    print(f"[gpu] merge queries {q}, with > 64 photons in the radius {over}, at most {mx}", flush=True)
+    assert over >= 100 and mx > 1000
+    _compare(g, o, "vcm dense merge")
