@@ -302,6 +302,15 @@ typedef enum lt_integrator_kind {
 #define LT_PATH_PERSISTENT 2u /* persistent lanes with path regeneration: a lane
                                  takes the next pixel as soon as its path ends */
 
+/* Weight of the Path integrator's next-event sample.  The reference divides
+ * by 1 / light_triangle_count whichever light was picked (path.rgen:76-78,
+ * each analytic light counting one; LumenScene.cpp:106), while it picks a
+ * light uniformly and then one of its triangles (commons.glsl:227-238): exact
+ * for a single emitter mesh, biased otherwise.  The corrected weight is the
+ * pick's own probability, 1 / (lights x the picked light's triangles). */
+#define LT_NEE_REFERENCE 0u
+#define LT_NEE_CORRECTED 1u
+
 /* Path integrator config: SceneConfig/PathConfig (src/RayTracer/SceneConfig.h:15-25)
  * plus the PCPath fields set by Path::render (src/RayTracer/Path.cpp:27-38). */
 typedef struct lt_path_config {
@@ -310,8 +319,10 @@ typedef struct lt_path_config {
     int32_t max_depth;        /* path_length (default 6)                 */
     float sky_col[3];
     uint32_t direct_lighting; /* Path.h: default 1                       */
-    uint32_t strategy;        /* LT_PATH_MEGAKERNEL or LT_PATH_WAVEFRONT  */
-    uint32_t reserved[6];
+    uint32_t strategy;        /* LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT or
+                                 LT_PATH_PERSISTENT                        */
+    uint32_t nee_pick;        /* LT_NEE_REFERENCE (default) or LT_NEE_CORRECTED */
+    uint32_t reserved[5];
 } lt_path_config;
 
 /* MIS heuristics of the BDPT integrator. */

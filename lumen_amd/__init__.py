@@ -333,8 +333,9 @@ class Path(Integrator):
     kind = LT_INTEGRATOR_PATH
 
     def __init__(self, lumen_scene, ctx, width=1920, height=1080, path_length=None, sky_col=None,
-                 direct_lighting=True, strategy="megakernel"):
+                 direct_lighting=True, strategy="megakernel", nee_pick="reference"):
         super().__init__(lumen_scene, ctx, width, height)
+        self.nee_pick = {"reference": _abi.LT_NEE_REFERENCE, "corrected": _abi.LT_NEE_CORRECTED}[nee_pick]
         icfg = lumen_scene.integrator_cfg()
         self.path_length = path_length if path_length is not None else (icfg.path_length or 6)
         self.sky_col = sky_col if sky_col is not None else tuple(icfg.sky_col)
@@ -349,6 +350,7 @@ class Path(Integrator):
         c.sky_col[:] = list(self.sky_col)
         c.direct_lighting = int(self.direct_lighting)
         c.strategy = self.strategy
+        c.nee_pick = self.nee_pick
         return c
 
 

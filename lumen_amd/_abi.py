@@ -26,6 +26,8 @@ LT_BSDF_FLAG_DIFFUSE, LT_BSDF_FLAG_SPECULAR, LT_BSDF_FLAG_GLOSSY, LT_BSDF_FLAG_R
 LT_LIGHT_SPOT, LT_LIGHT_AREA, LT_LIGHT_DIRECTIONAL, LT_LIGHT_POINT = 1, 2, 3, 4
 LT_LIGHT_FINITE, LT_LIGHT_DELTA = 1 << 4, 1 << 5
 LT_PTR_DEVICE = 1
+LT_NEE_REFERENCE = 0
+LT_NEE_CORRECTED = 1
 LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM = 0, 1, 2
 LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT, LT_PATH_PERSISTENT = 0, 1, 2
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
@@ -91,7 +93,7 @@ class lt_build_stats(C.Structure):
 class lt_path_config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
                 ("sky_col", C.c_float * 3), ("direct_lighting", C.c_uint32), ("strategy", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("nee_pick", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
 class lt_bdpt_config(C.Structure):

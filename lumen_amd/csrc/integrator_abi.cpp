@@ -67,6 +67,8 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
         if (pc->strategy != LT_PATH_MEGAKERNEL && pc->strategy != LT_PATH_WAVEFRONT &&
             pc->strategy != LT_PATH_PERSISTENT)
             return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown path strategy");
+        if (pc->nee_pick != LT_NEE_REFERENCE && pc->nee_pick != LT_NEE_CORRECTED)
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown nee_pick");
         if (pc->strategy == LT_PATH_WAVEFRONT && pc->max_depth > kWaveMaxDepth)
             return lt_fail(ctx, LT_ERR_UNSUPPORTED, "wavefront path length is limited to 32");
         W = pc->width;
@@ -191,6 +193,7 @@ static lt_status launch_frames(lt_integrator* it, const lt_frame_params* p, uint
     L.max_depth = it->path.max_depth;
     memcpy(L.sky_col, it->path.sky_col, 12);
     L.direct_lighting = it->path.direct_lighting;
+    L.nee_pick = it->path.nee_pick;
     L.frame_start = frame_num;
     L.spp = spp;
     L.seed = p->seed;

@@ -51,7 +51,7 @@ struct NeeRays {
 };
 
 LT_DEV void nee_sample(const SceneView& sv, RNG& seed, const Mat& mat, const Payload& hp, bool side, f3 n_s, f3 wo,
-                       f3 scale, NeeRays& r) {
+                       f3 thr, uint32_t nee_pick, NeeRays& r) {
     f3 wi;
     float wi_len, pdf_light_w, pdf_light_a, cos_from_light;
     LightRecord rec;
@@ -63,6 +63,7 @@ LT_DEV void nee_sample(const SceneView& sv, RNG& seed, const Mat& mat, const Pay
     r4.z = rnd(seed);
     r4.w = rnd(seed);
     const f3 Le = sample_light_Li(sv, r4, hp.pos, pdf_light_w, wi, wi_len, pdf_light_a, cos_from_light, rec);
+    const f3 scale = thr / nee_pick_pdf(sv, nee_pick, rec.light_idx);
     r.org = offset_ray2(hp.pos, n_s);
     r.wi = wi;
     r.tmax = 0.0f;
@@ -183,8 +184,7 @@ LT_DEV bool path_bounce(const SceneView& sv, const PathLaunch& L, PathState& p, 
         p.origin = offset_ray(hp.pos, n_g);
         p.last_specular = is_specular(hit_mat);
         if (!p.last_specular && (p.depth > 0 || L.direct_lighting == 1)) {
-            const float light_pick_pdf = 1.0f / (float)sv.total_light_triangle_cnt;
-            nee_sample(sv, p.seed, hit_mat, hp, side, n_s, wo, p.throughput / light_pick_pdf, nee);
+            nee_sample(sv, p.seed, hit_mat, hp, side, n_s, wo, p.throughput, L.nee_pick, nee);
             has_nee = true;
         }
         float pdf, cos_theta;

@@ -152,7 +152,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LT_WF_SHADE
                 const bool spec = is_specular(hit_mat);
                 last_spec = spec ? 1u : 0u;
                 if (!spec && (depth > 0 || L.direct_lighting == 1)) {
-                    const float lpp = 1.0f / (float)sv.total_light_triangle_cnt;
                     f3 wi;
                     float wi_len, pdf_light_w, pdf_light_a, cos_from_light;
                     LightRecord rec;
@@ -165,6 +164,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LT_WF_SHADE
                     r4.w = rnd(seed);
                     const f3 Le =
                         sample_light_Li(sv, r4, hp.pos, pdf_light_w, wi, wi_len, pdf_light_a, cos_from_light, rec);
+                    const float lpp = nee_pick_pdf(sv, L.nee_pick, rec.light_idx);
                     const f3 ps = offset_ray2(hp.pos, n_s);
                     float bsdf_pdf, bsdf_rev;
                     float cos_x = dot(n_s, wi);

@@ -27,7 +27,7 @@ class PathParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
                 ("sky_col", C.c_float * 3), ("direct_lighting", C.c_uint32), ("frame_start", C.c_uint32),
                 ("frames", C.c_uint32), ("seed", C.c_uint32), ("x0", C.c_uint32), ("y0", C.c_uint32),
-                ("x1", C.c_uint32), ("y1", C.c_uint32)]
+                ("x1", C.c_uint32), ("y1", C.c_uint32), ("nee_pick", C.c_uint32)]
 
 
 class BdptParams(C.Structure):
@@ -242,9 +242,11 @@ class Accel:
         return out
 
     def render_path(self, camera, width, height, max_depth, sky_col=(0, 0, 0), direct_lighting=1, frame_start=0,
-                    frames=1, seed=0, window=None, accum=None, threads=0):
-        """Runs the reference Path integrator on the CPU; returns (image, seconds, closest, shadow)."""
+                    frames=1, seed=0, window=None, accum=None, threads=0, nee_pick=0):
+        """Runs the reference Path integrator on the CPU; returns (image, seconds, closest, shadow).
+        nee_pick=1: the NEE sample is weighted by its own pick probability."""
         p = PathParams()
+        p.nee_pick = nee_pick
         p.width, p.height, p.max_depth = width, height, max_depth
         p.sky_col[:] = list(sky_col)
         p.direct_lighting = direct_lighting

@@ -232,6 +232,7 @@ struct PathParams {
     vec3 sky_col;
     uint32_t direct_lighting;
     uint32_t frame_num;  // RNG frame word
+    uint32_t nee_pick;   // 0: the reference's 1 / light_triangle_count, 1: the sampled light's own pick pdf
 };
 struct Counters {
     uint64_t closest = 0, shadow = 0;

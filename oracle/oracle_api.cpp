@@ -173,6 +173,7 @@ typedef struct ora_path_params {
     uint32_t frames;       // frames to accumulate
     uint32_t seed;         // XOR'ed into the frame word (0 = reference)
     uint32_t x0, y0, x1, y1;  // pixel window [x0,x1) x [y0,y1); x1 = 0 means full width/height
+    uint32_t nee_pick;        // 0: the reference's light pick weight, 1: the pick's own probability
 } ora_path_params;
 
 // accum: RGBA32F, row-major over the FULL image; returns wall seconds
@@ -190,6 +191,7 @@ double ora_render_path(const lt_scene_desc* s, void* accel, const lt_camera* cam
         pp.width = W;
         pp.height = H;
         pp.max_depth = p->max_depth;
+        pp.nee_pick = p->nee_pick;
         pp.sky_col = load3(p->sky_col);
         pp.direct_lighting = p->direct_lighting;
         pp.frame_num = frame_num ^ p->seed;

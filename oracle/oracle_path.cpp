@@ -33,7 +33,7 @@ static void trace_closest(const lt_scene_desc& s, const Accel* a, vec3 o, vec3 d
 
 // pt_commons.glsl:3-42
 static vec3 uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& seed, const Material& mat, vec3 pos,
-                                 bool side, vec3 n_s, vec3 wo, PState& st, Counters* cnt) {
+                                 bool side, vec3 n_s, vec3 wo, PState& st, Counters* cnt, float& pick_pdf) {
     vec3 res = v3(0.0f);
     vec3 wi;
     float wi_len, pdf_light_w, pdf_light_a, cos_from_light, pdf_dir;
@@ -46,6 +46,11 @@ static vec3 uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& 
     r4.w = rand(seed);
     const vec3 Le = sample_light_Li(s, r4, pos, (int)s.num_lights, pdf_light_w, wi, wi_len, pdf_light_a,
                                     cos_from_light, rec, ln, lpos, pdf_dir);
+    // probability of the pick made above: a light uniformly (commons.glsl:227),
+    // then, for an area light, one of its triangles uniformly (commons.glsl:238)
+    const lt_light& picked = s.lights[rec.light_idx];
+    pick_pdf = 1.0f / ((float)s.num_lights *
+                       ((picked.light_flags & 7u) == LT_LIGHT_AREA ? (float)picked.num_triangles : 1.0f));
     const vec3 p = offset_ray2(pos, n_s);
     float bsdf_pdf, bsdf_rev;
     float cos_x = dot(n_s, wi);
@@ -119,10 +124,15 @@ vec3 path_sample(const lt_scene_desc& s, const Accel* a, const lt_camera& cam, c
         origin = offset_ray(hp.pos, n_g);
         last_specular = is_specular(hit_mat);
         if (!last_specular) {
-            const float light_pick_pdf = 1.0f / (float)s.total_light_triangle_cnt;
+            // path.rgen:76-78: the reference divides by 1 / light_triangle_count
+            // whichever light was picked (unbiased for one emitter mesh only);
+            // nee_pick = 1 divides by the pick's own probability instead
+            float light_pick_pdf = 1.0f / (float)s.total_light_triangle_cnt;
             if (depth > 0 || pp.direct_lighting == 1) {
-                col += throughput * uniform_sample_light(s, a, seed, hit_mat, hp.pos, side, n_s, wo, st, cnt) /
-                       light_pick_pdf;
+                float pick_pdf;
+                const vec3 nee = uniform_sample_light(s, a, seed, hit_mat, hp.pos, side, n_s, wo, st, cnt, pick_pdf);
+                if (pp.nee_pick == 1) light_pick_pdf = pick_pdf;
+                col += throughput * nee / light_pick_pdf;
             }
         }
         float pdf, cos_theta;

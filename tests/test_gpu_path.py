@@ -90,3 +90,25 @@ def test_execution_forms_equal_megakernel(gpu_ctx, kind, W, H, frames, kw, strat
     assert np.all(np.isfinite(b))
     d = np.abs(a[..., :3] - b[..., :3])
     assert np.all(d <= 1e-4 * (1.0 + np.abs(a[..., :3]))), d.max()
+
+
+@pytest.mark.parametrize("strategy", ["megakernel", "persistent", "wavefront"])
+def test_corrected_nee_pick_replays_oracle(gpu_ctx, strategy):
+    """LT_NEE_CORRECTED: the NEE sample weighted by its own pick probability,
+    on the terrain's 8 point lights + 1 two-triangle area light."""
+    print(f"[gpu] corrected nee pick {strategy}", flush=True)
+    sc = la.Scene.procedural("terrain", seed=1, terrain_tris=20000, num_instances=100, num_point_lights=8)
+    W, H = 64, 36
+    it = la.Path(sc, gpu_ctx, W, H, strategy=strategy, nee_pick="corrected")
+    it.init()
+    it.create_accel()
+    it.render(spp=2)
+    g = it.output_tex
+    it.destroy()
+    o, _, _, _ = oracle.Accel(sc.desc).render_path(it.camera, W, H, it.path_length, sky_col=tuple(it.sky_col),
+                                                   frames=2, nee_pick=1)
+    d = np.abs(g[..., :3] - o[..., :3])
+    ok = np.all(d <= 1e-3 * (1.0 + np.abs(o[..., :3])), axis=-1)
+    rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
+    print(f"[gpu] corrected nee {strategy}: pixels matching {ok.mean():.4f}, rel rmse {rel:.2e}", flush=True)
+    assert ok.mean() >= 0.97 and rel <= 0.02

@@ -56,3 +56,33 @@ def test_nan_guard_and_running_mean():
     for f in range(3):
         b, _, _, _ = acc.render_path(cam, 8, 8, 4, frame_start=f, frames=1, accum=b)
     assert np.array_equal(a, b)
+
+
+def test_nee_pick_reference_bias_and_corrected_weight():
+    """A point light and a two-triangle area light over a Lambert plane, Path
+    at depth 2 (direct light only).  Rendered alone, each light is estimated
+    without bias by the reference's weight (one emitter).  Together the
+    reference weight is light_triangle_count = 3 for every pick (path.rgen:76-78),
+    while the pick probabilities are 1/2 (point) and 1/4 (each area triangle):
+    the point light comes out 3/2 too bright and the area light 3/4 too dark.
+    The corrected weight (nee_pick=1) gives the sum of the two."""
+    rho = 0.6
+
+    def scene(point, area):
+        s = DescScene()
+        m = s.material(bsdf_type=1, bsdf_props=9, albedo=(rho,) * 3)
+        s.mesh(quad([-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]), m)
+        if area:
+            lm = s.material(bsdf_type=1, bsdf_props=9, albedo=(0.0,) * 3, emissive_factor=(8.0, 8.0, 8.0))
+            s.mesh(quad([-1.5, 2.0, -0.5], [-0.5, 2.0, -0.5], [-0.5, 2.0, 0.5], [-1.5, 2.0, 0.5]), lm)
+        if point:
+            s.analytic.append(dict(pos=(1.0, 2.5, 0.0), to=(0, 0, 0), L=(12.0,) * 3, flags=4 | 16 | 32))
+        return oracle.Accel(s.build())
+
+    cam = camera([0.0, 10.0, 0.0], [0.0, -1.0, 0.0], 1.0)
+    render = lambda acc, pick: acc.render_path(cam, 5, 5, 2, frames=2048, nee_pick=pick)[0][2, 2, 0]
+    p_only = render(scene(True, False), 0)
+    a_only = render(scene(False, True), 0)
+    both = scene(True, True)
+    assert render(both, 1) == pytest.approx(p_only + a_only, rel=0.02)
+    assert render(both, 0) == pytest.approx(1.5 * p_only + 0.75 * a_only, rel=0.02)
