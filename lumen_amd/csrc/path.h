@@ -30,3 +30,8 @@ LT_DEV inline float nee_pick_pdf(const SceneView& sv, uint32_t nee_pick, uint32_
     const lt_light& l = sv.lights[li];
     return 1.0f / ((float)sv.num_lights * ((l.light_flags & 7u) == LT_LIGHT_AREA ? (float)l.num_triangles : 1.0f));
 }
+
+// persistent lanes with path regeneration, one launch per frame
+lt_status launch_path_persistent(lt_context* ctx, const PathLaunch& L, float4* accum, int count, uint64_t* d_counts);
+lt_status launch_path_megakernel(lt_context* ctx, const PathLaunch& L, float4* accum, int count,
+                                 uint64_t* d_counts);
