@@ -61,8 +61,8 @@ def test_cli_writes_exr(gpu_ctx):
 
 @pytest.mark.parametrize("name,extra,py_kw", [
     ("bdpt", ["--mis", "power"], dict(mis="power")),
-    ("vcm", ["--estimator", "corrected"], dict(estimator="corrected", enable_vm=True)),
-    ("vcm", [], dict(estimator="reference", enable_vm=True)),
+    ("vcm", ["--estimator", "corrected"], dict(estimator="corrected")),
+    ("vcm", [], dict(estimator="reference")),
 ])
 def test_cli_bdpt_vcm_match_python_integrators(gpu_ctx, name, extra, py_kw):
     """The C++ factory builds BDPT and VCM (RayTracer::create_integrator,

@@ -119,7 +119,7 @@ def test_terrain_path_frame_band(gpu_ctx, terrain):
     print("[gpu] terrain path: oracle band", flush=True)
     o, secs, _, _ = acc.render_path(it.camera, W, H, it.path_length, sky_col=tuple(it.sky_col), frames=1,
                                     window=(0, y0, W, y1))
-    _check_image(g[y0:y1], o[y0:y1], "terrain path 1080p band", 0.995, 0.05)
+    _check_image(g[y0:y1], o[y0:y1], "terrain path 1080p band", 0.999, 0.01)
 
 
 def test_terrain_execution_forms_full_frame(gpu_ctx, terrain):
@@ -164,7 +164,7 @@ def test_caustics_bdpt_power_full_frame(gpu_ctx, caustics):
     it.destroy()
     print("[gpu] caustics bdpt: oracle frame", flush=True)
     o, _, _, _ = acc.render_bdpt(it.camera, W, H, 8, frames=1, seed=7, power=True)
-    _check_image(g, o, "caustics bdpt power 1024^2", 0.995, 0.03)
+    _check_image(g, o, "caustics bdpt power 1024^2", 0.999, 0.02)
 
 
 @pytest.mark.parametrize("estimator", ["corrected", "reference"])
@@ -186,7 +186,7 @@ def test_caustics_vcm_full_frame(gpu_ctx, caustics, estimator):
         q, over, mx = oracle.vcm_merge_stats()
         print(f"[gpu] merge queries {q}, with > 64 photons in the radius {over}, at most {mx}", flush=True)
         assert over > 1000 and mx > 10000  # the dense caustic: the merge kernel's queue carries over
-    _check_image(g, o, f"caustics vcm {estimator} 1024^2", 0.99, 0.03)
+    _check_image(g, o, f"caustics vcm {estimator} 1024^2", 0.999, 0.01)
 
 
 # ------------------------------------------------------------------ adjoint
@@ -226,12 +226,12 @@ def test_adjoint_c4_two_variants(gpu_ctx, diff_wall):
     o_img = acc.adjoint(adj.camera, W, H, depth, spp, **kw)
     o_ga, o_gr = acc.adjoint(adj.camera, W, H, depth, spp, d_images=w, **kw)
     for v in range(V):
-        _check_image(g_img[v], o_img[v], f"adjoint c4 forward v{v}", 0.995, 0.02)
+        _check_image(g_img[v], o_img[v], f"adjoint c4 forward v{v}", 0.999, 0.005)
     for g, o, tag in ((ga, o_ga, "albedo"), (gr, o_gr, "roughness")):
         rel = np.linalg.norm(g - o) / np.linalg.norm(o)
         ok = np.abs(g - o) <= 1e-3 * np.abs(o).max() + 1e-2 * np.abs(o)
         print(f"[gpu] adjoint c4 {tag} gradient: rel L2 {rel:.2e}, texels matching {ok.mean():.5f}", flush=True)
-        assert rel <= 1e-2 and ok.mean() >= 0.995, tag
+        assert rel <= 1e-3 and ok.mean() >= 0.999, tag
 
 
 def test_adjoint_gradients_match_float64_central_differences(gpu_ctx, diff_wall):
