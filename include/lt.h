@@ -239,6 +239,17 @@ lt_status lt_accel_export(lt_context* ctx, void* nodes, size_t nodes_bytes, void
  * ray.rchit / ray.rmiss identity fields (src/shaders/ray.rchit:24-84,
  * ray.rmiss:13).  Ties in t resolve to the smaller (inst, prim). */
 lt_status lt_trace_closest(lt_context* ctx, const lt_ray* rays, size_t n, lt_hit* hits, uint32_t flags);
+/* Closest hits like lt_trace_closest with LT_PTR_DEVICE (rays and hits are
+ * device pointers; asynchronous on the context stream), for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * It calls no hipMalloc / hipFree: the persistent grid's work counter and its
+ * traversal-stack spill area are a fixed device array of 8 MiB + 256 B per
+ * device (at most 1024 workgroups of 128 threads, 16 spill entries each),
+ * held until the process ends and shared by every call on that device, so
+ * calls that overlap on one GPU race.  A BVH deeper than 33 levels (more than
+ * the 16 LDS + 16 spill stack entries) is refused with LT_ERR_UNSUPPORTED. */
+lt_status lt_trace_closest_noalloc(lt_context* ctx, const lt_ray* rays, size_t n, lt_hit* hits);
 /* Occlusion: traceRayEXT(TerminateOnFirstHit|SkipClosestHit) + ray.rahit /
  * ray_shadow.rmiss (src/shaders/integrators/pt_commons.glsl:20-23,
  * ray.rahit:18-20, ray_shadow.rmiss:12).  occluded[i] = 1 iff any triangle is

@@ -155,6 +155,7 @@ _PROTOS = {
     "lt_accel_build": (C.c_int, [P, C.POINTER(lt_build_opts), C.POINTER(lt_build_stats)]),
     "lt_accel_export": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),
     "lt_trace_closest": (C.c_int, [P, P, C.c_size_t, P, C.c_uint32]),
+    "lt_trace_closest_noalloc": (C.c_int, [P, P, C.c_size_t, P]),
     "lt_trace_any": (C.c_int, [P, P, C.c_size_t, P, C.c_uint32]),
     "lt_trace_count": (C.c_int, [P, P, C.c_size_t, C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint64)]),

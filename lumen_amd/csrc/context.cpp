@@ -109,6 +109,10 @@ lt_status lt_context_create(int device, lt_context** out) {
     }
     ctx->num_cus = prop.multiProcessorCount;
     ctx->arch = prop.gcnArchName;
+    if (noalloc_scratch_init(ctx) != LT_OK) {
+        lt_context_destroy(ctx);
+        return LT_ERR_HIP;
+    }
     *out = ctx;
     return LT_OK;
 }

@@ -86,6 +86,9 @@ struct lt_context {
     void* hit_buf = nullptr;
     size_t hit_buf_bytes = 0;
     uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
+    // device address of the no-allocation entry point's scratch array (shared
+    // by every context on this device; not owned)
+    uint32_t* noalloc_scratch = nullptr;
     float last_ms = 0.0f;
     lt_tuning tuning{10u, 8u, 32u, 0u, 4u, {}};
     // bumped whenever device state that launches capture by pointer or value
@@ -117,3 +120,5 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
 lt_status launch_frame_word(lt_context* ctx, uint32_t* p, uint32_t value, int advance, hipStream_t stream);
 lt_status launch_trace(lt_context* ctx, const lt_ray* d_rays, size_t n, void* d_out, int any_hit, int count,
                        uint64_t* d_counts);
+// resolves ctx->noalloc_scratch (once per device)
+lt_status noalloc_scratch_init(lt_context* ctx);
