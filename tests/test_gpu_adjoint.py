@@ -4,10 +4,10 @@ oracle's differentiable Path estimator and its adjoint
 
 Same scene, camera, seed and texels: each sample replays the oracle's pcg4d
 stream, so the images agree to float tolerance (the Path replay bar:
-|gpu - cpu| <= 1e-3 (1 + |cpu|) on >= 97% of the pixels, relative RMSE <=
-2%).  Gradients are sums of many float atomics on the GPU and of doubles on
-the host; they must agree to a relative L2 error of 1e-2 with >= 97% of the
-texels within 1e-3 * max|g| + 1e-2 |g|.
+|gpu - cpu| <= 1e-3 (1 + |cpu|) on >= 99.5% of the pixels, relative RMSE
+<= 1%).  Gradients are sums of many float atomics on the GPU and of doubles
+on the host; they must agree to a relative L2 error of 1e-4 with >= 99.9% of
+the texels within 1e-4 * max|g| + 1e-3 |g| (measured: about 1e-6).
 """
 import numpy as np
 import pytest
@@ -42,11 +42,11 @@ def _cmp_grad(g, o, tag):
     o = np.asarray(o, np.float64)
     assert np.all(np.isfinite(g))
     err = np.abs(g - o)
-    ok = err <= 1e-3 * np.abs(o).max() + 1e-2 * np.abs(o)
+    ok = err <= 1e-4 * np.abs(o).max() + 1e-3 * np.abs(o)
     rel = np.linalg.norm(g - o) / max(np.linalg.norm(o), 1e-30)
     print(f"[gpu] {tag}: texels matching {ok.mean():.4f}, rel L2 {rel:.2e}", flush=True)
-    assert ok.mean() >= 0.97
-    assert rel <= 1e-2
+    assert ok.mean() >= 0.999
+    assert rel <= 1e-4
 
 
 @pytest.mark.parametrize("W,H,V,spp,depth", [(24, 24, 2, 2, 4), (40, 32, 3, 2, 6)])
@@ -81,7 +81,11 @@ def test_adjoint_replays_oracle(gpu_ctx, W, H, V, spp, depth):
 def test_adjoint_albedo_central_differences_on_gpu(gpu_ctx):
     """h = 1e-3 central differences of the GPU forward itself on 64 random
     texels (paths do not depend on albedo: the estimator is a polynomial in
-    it, so the differences match to float rounding)."""
+    it, so the differences match to float rounding).  In float32 the
+    difference of two images carries about 1e-7 of each pixel, i.e. 5e-5 of
+    it after dividing by 2h, hence the loose bar here; the 1e-3 contract is
+    checked against float64 central differences of the oracle's estimator
+    (test_gpu_fullsize.py)."""
     print("[gpu] adjoint albedo central differences", flush=True)
     from lumen_amd.adjoint import Adjoint
     W = H = 32

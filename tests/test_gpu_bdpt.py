@@ -8,8 +8,8 @@ the oracle multiplies them out per strategy) and the order in which splats of
 other pixels' light subpaths are summed (float atomics).  Transcendentals
 differ in the last ulp between the device library and libm, which flips a
 rare discrete decision.  The bar is therefore per-pixel
-|gpu - cpu| <= 1e-3 * (1 + |cpu|) on >= 97% of the pixels and a relative
-image RMSE <= 2%.
+|gpu - cpu| <= 1e-3 * (1 + |cpu|) on >= 99.5% of the pixels and a relative
+image RMSE <= 1% (measured: at least 99.7% and at most 0.7%).
 """
 import numpy as np
 import pytest
@@ -45,8 +45,8 @@ def _compare(g, o, tag):
     ok = np.all(d <= 1e-3 * (1.0 + np.abs(o[..., :3])), axis=-1)
     rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
     print(f"[gpu] {tag}: pixels matching {ok.mean():.4f}, rel rmse {rel:.2e}", flush=True)
-    assert ok.mean() >= 0.97
-    assert rel <= 0.02
+    assert ok.mean() >= 0.995
+    assert rel <= 0.01
 
 
 def _render_both(ctx, sc, W, H, frames, depth, mis="balance", seed=0):

@@ -4,8 +4,8 @@ Same scene, camera, seed and frame indices: the per-pixel pcg4d stream is a
 pure function of (x, y, frame) (path.rgen:23), so the GPU image replays the
 oracle's.  Transcendentals (cos/sin/pow) differ in the last ulp between the
 device library and libm, which flips a rare discrete decision; the bar is
-therefore per-pixel |gpu - cpu| <= 1e-3 * (1 + |cpu|) on >= 97% of the pixels
-and a relative image RMSE <= 2%.
+therefore per-pixel |gpu - cpu| <= 1e-3 * (1 + |cpu|) on >= 99.5% of the
+pixels and a relative image RMSE <= 1%.
 """
 import numpy as np
 import pytest
@@ -44,8 +44,8 @@ def test_path_replays_oracle(gpu_ctx, kind, W, H, frames, kw):
     ok = np.all(d <= 1e-3 * (1.0 + np.abs(o[..., :3])), axis=-1)
     rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
     print(f"[gpu] {kind}: pixels matching {ok.mean():.4f}, rel rmse {rel:.2e}", flush=True)
-    assert ok.mean() >= 0.97
-    assert rel <= 0.02
+    assert ok.mean() >= 0.995
+    assert rel <= 0.01
 
 
 def test_running_mean_equals_single_launch(gpu_ctx):
@@ -111,4 +111,4 @@ def test_corrected_nee_pick_replays_oracle(gpu_ctx, strategy):
     ok = np.all(d <= 1e-3 * (1.0 + np.abs(o[..., :3])), axis=-1)
     rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
     print(f"[gpu] corrected nee {strategy}: pixels matching {ok.mean():.4f}, rel rmse {rel:.2e}", flush=True)
-    assert ok.mean() >= 0.97 and rel <= 0.02
+    assert ok.mean() >= 0.995 and rel <= 0.01

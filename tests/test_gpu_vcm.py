@@ -11,7 +11,7 @@ which float atomics sum the splats (and, for the corrected estimator, the
 order of photons within a bucket), and last-ulp differences of the device
 transcendentals, which flip a rare discrete decision.  The bar is the one of
 the Path and BDPT replays: per-pixel |gpu - cpu| <= 1e-3 * (1 + |cpu|) on
->= 97% of the pixels and a relative image RMSE <= 2%.
+>= 99.5% of the pixels and a relative image RMSE <= 1%.
 """
 import numpy as np
 import pytest
