@@ -43,6 +43,11 @@ class VcmParams(C.Structure):
                 ("corrected", C.c_uint32), ("sky_col", C.c_float * 3)]
 
 
+class SppmParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("frame_start", C.c_uint32),
+                ("frames", C.c_uint32), ("seed", C.c_uint32), ("base_radius", C.c_float), ("sky_col", C.c_float * 3)]
+
+
 class AdjParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("spp", C.c_uint32),
                 ("seed", C.c_uint32), ("sky_col", C.c_float * 3), ("albedo_id", C.c_int32), ("rough_id", C.c_int32),
@@ -102,6 +107,13 @@ def lib() -> C.CDLL:
         L.ora_vcm_radius_grid.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_float),
                                           C.POINTER(C.c_int32)]
         L.ora_hardware_threads.restype = C.c_int
+        L.ora_sppm_create.argtypes = [C.c_uint32, C.c_uint32]
+        L.ora_sppm_create.restype = P
+        L.ora_sppm_free.argtypes = [P]
+        L.ora_sppm_word.argtypes = [C.c_uint32, C.c_uint32]
+        L.ora_sppm_word.restype = C.c_uint32
+        L.ora_render_sppm.argtypes = [P, P, P, C.POINTER(SppmParams), P, P, P, C.c_int, C.POINTER(C.c_uint64)]
+        L.ora_render_sppm.restype = C.c_double
         _lib = L
     return _lib
 
@@ -299,6 +311,26 @@ class Accel:
                                     C.byref(sh))
         return accum, secs, cl.value, sh.value
 
+    def render_sppm(self, camera, width, height, max_depth, frame_start=0, frames=1, seed=0, base_radius=0.03,
+                    sky_col=(0, 0, 0), state=None, accum=None, threads=0):
+        """Runs the reference SPPM integrator on the CPU; returns (image,
+        seconds, rays, grid [min3, max3, res3], state).  Pass the returned
+        state back to continue at frame_start > 0."""
+        p = SppmParams()
+        p.width, p.height, p.max_depth = width, height, max_depth
+        p.frame_start, p.frames, p.seed, p.base_radius = frame_start, frames, seed, base_radius
+        p.sky_col[:] = list(sky_col)
+        if state is None:
+            state = SppmState(width, height)
+        if accum is None:
+            accum = np.zeros((height, width, 4), np.float32)
+        grid = np.zeros(9, np.float32)
+        rays = C.c_uint64()
+        secs = lib().ora_render_sppm(C.byref(self.desc), self._h, C.byref(camera), C.byref(p), state.handle,
+                                     accum.ctypes.data, grid.ctypes.data, threads or lib().ora_hardware_threads(),
+                                     C.byref(rays))
+        return accum, secs, rays.value, grid, state
+
     def vcm_grid_selfcheck(self, camera, width, height, max_depth, radius_factor, nq=200, seed=0):
         p = VcmParams()
         p.width, p.height, p.max_depth = width, height, max_depth
@@ -411,6 +443,23 @@ class Accel64:
             lib64().ora64_accel_free(self._h)
         except Exception:
             pass
+
+
+class SppmState:
+    """Per-pixel SPPM data carried between frames (sppm_commons.h:27-42)."""
+
+    def __init__(self, width, height):
+        self.handle = lib().ora_sppm_create(width, height)
+
+    def __del__(self):
+        try:
+            lib().ora_sppm_free(self.handle)
+        except Exception:
+            pass
+
+
+def sppm_word(seed: int, frame_num: int) -> int:
+    return lib().ora_sppm_word(seed, frame_num)
 
 
 def bvh8_trace(nodes: np.ndarray, tris: np.ndarray, rays: np.ndarray, any_hit: bool = False):

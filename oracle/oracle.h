@@ -237,6 +237,9 @@ struct PathParams {
 struct Counters {
     uint64_t closest = 0, shadow = 0;
 };
+// pt_commons.glsl:3-42 at a vertex with closest-hit payload `hit`
+vec3 nee_uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& seed, const Material& mat, vec3 pos,
+                              bool side, vec3 n_s, vec3 wo, const Payload& hit, Counters* cnt);
 // One sample of pixel (x, y).  `nan_dropped` reports the reference's NaN guard.
 vec3 path_sample(const lt_scene_desc& s, const Accel* a, const lt_camera& cam, const PathParams& pp, uint32_t x,
                  uint32_t y, bool& nan_dropped, Counters* cnt);

@@ -80,6 +80,18 @@ static vec3 uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& 
     return res;
 }
 
+// uniform_sample_light of pt_commons.glsl:3-42 at a vertex whose closest-hit
+// payload is `hit` (what a missed MIS ray compares, ray.rmiss:13), for the
+// other integrators that call it (SPPM's eye pass).
+vec3 nee_uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& seed, const Material& mat, vec3 pos,
+                              bool side, vec3 n_s, vec3 wo, const Payload& hit, Counters* cnt) {
+    PState st{};
+    st.pl = hit;
+    st.hit_valid = true;
+    float pick;
+    return uniform_sample_light(s, a, seed, mat, pos, side, n_s, wo, st, cnt, pick);
+}
+
 // path.rgen:26-104
 vec3 path_sample(const lt_scene_desc& s, const Accel* a, const lt_camera& cam, const PathParams& pp, uint32_t x,
                  uint32_t y, bool& nan_dropped, Counters* cnt) {
