@@ -49,7 +49,7 @@ def traffic_from_profile(workload: str, integrator: str):
     return None
 
 
-def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float):
+def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float, power: bool = False):
     import oracle
 
     log("cpu baseline: building host BVH")
@@ -57,7 +57,7 @@ def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float):
     threads = oracle.hardware_threads()
     y0 = H // 2 - 16
     _, secs, _, _ = acc.render_bdpt(camera, W, H, depth, frame_start=frame, frames=1, window=(0, y0, W, y0 + 32),
-                                    threads=threads)
+                                    threads=threads, power=power)
     per_row = max(secs / 32, 1e-5)
     if per_row * H >= target_s:
         rows, frames = int(max(8, target_s / per_row)), 1
@@ -66,7 +66,7 @@ def cpu_baseline_bdpt(scene, camera, W, H, depth, frame, target_s: float):
     y0 = max(0, H // 2 - rows // 2)
     log(f"cpu baseline: {rows} rows x {W} px x {frames} frame(s), {threads} threads")
     _, secs, cl, sh = acc.render_bdpt(camera, W, H, depth, frame_start=frame, frames=frames,
-                                      window=(0, y0, W, y0 + rows), threads=threads)
+                                      window=(0, y0, W, y0 + rows), threads=threads, power=power)
     return {
         "value": round((cl + sh) / secs / 1e6, 4),
         "unit": "Mrays/s",
@@ -99,6 +99,30 @@ def cpu_baseline_vcm(scene, camera, W, H, depth, frame, target_s: float, vcm_kw:
         "kind": "port",
         "sample": (f"{frames} full {W}x{H} frame(s) {frame}..{frame + frames - 1} (light and eye passes, "
                    f"{cl_all + sh_all} rays, {total_s:.2f} s)"),
+    }
+
+
+def cpu_baseline_sppm(scene, camera, W, H, depth, target_s: float, base_radius: float):
+    import oracle
+
+    log("cpu baseline: building host BVH")
+    acc = oracle.Accel(scene.desc)
+    threads = oracle.hardware_threads()
+    frames, total_s, rays, state, img = 0, 0.0, 0, None, None
+    while total_s < target_s and frames < 16:  # whole frames from frame 0: the per-pixel state carries over
+        img, secs, r, _, state = acc.render_sppm(camera, W, H, depth, frame_start=frames, frames=1, seed=7,
+                                                 base_radius=base_radius, state=state, accum=img, threads=threads)
+        frames += 1
+        total_s += secs
+        rays += r
+    log(f"cpu baseline: {frames} full frame(s), {threads} threads")
+    return {
+        "value": round(rays / total_s / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{frames} full {W}x{H} frame(s) 0..{frames - 1} (eye, light and gather passes, {rays} rays, "
+                  f"{total_s:.2f} s)",
     }
 
 
@@ -146,10 +170,14 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
     Lumen's RMSE is sqrt(S) / (3 N) (N pixels); the standard RMSE is
     sqrt(S / (3 N)), taken relative to the ground truth's mean.  The first
     frame count meeting each bound times the measured ms per frame is the
-    time.  Where the standard bound lies beyond the frames checked, the
-    frame count comes from the fitted sigma^2 of RMSE^2(k) = sigma^2 (1/k +
-    1/gt_frames) (the ground truth's own variance included) and is marked
-    extrapolated."""
+    time.  A bound not met within the frames checked reports frames = null
+    with the last measured value; nothing is extrapolated.  Beside it, the
+    tail of the measurements (the checks at >= 1/4 of the last frame count)
+    is fitted to RMSE^2(k) = sigma^2 (1/k + 1/gt_frames), the ground truth's
+    own variance included: the fit's prediction at the last check, the noise
+    floor sqrt(sigma^2 / gt_frames) that a ground truth of gt_frames frames
+    leaves, and the frames a noise-free ground truth would need
+    (sigma^2 / tol^2), given only when that floor lies below the bound."""
     import numpy as np
 
     seed0, frame0 = it.seed, it.frame_num
@@ -187,15 +215,16 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
         if (lumen_k is not None and std <= tol * mean) or time.perf_counter() - t_check > budget_s:
             break
     ks = np.array([p[0] for p in pts], np.float64)
-    std2 = np.array([p[2] for p in pts], np.float64) ** 2
-    basis = 1.0 / ks + 1.0 / gt_frames
-    sigma2 = float((basis * std2).sum() / (basis * basis).sum())
-    target2 = (tol * mean) ** 2
+    rel2 = (np.array([p[2] for p in pts], np.float64) / mean) ** 2
     std_k = next((p[0] for p in pts if p[2] <= tol * mean), None)
-    extrapolated = std_k is None
-    if extrapolated:
-        # sigma^2 / k = (tol * mean)^2: the frames of a noise-free ground truth
-        std_k = round(sigma2 / target2, 1)
+    tail = ks >= ks[-1] / 4
+    basis = 1.0 / ks[tail] + 1.0 / gt_frames
+    sigma2 = float((basis * rel2[tail]).sum() / (basis * basis).sum())
+    floor_rel = float(np.sqrt(sigma2 / gt_frames))
+    fit = {"points": int(tail.sum()), "sigma2_rel_per_frame": sigma2,
+           "predicted_at_max_checked": float(np.sqrt(sigma2 * (1.0 / ks[-1] + 1.0 / gt_frames))),
+           "gt_noise_floor_rel": floor_rel,
+           "frames_with_noise_free_gt": round(sigma2 / tol ** 2, 1) if floor_rel < tol else None}
     it.seed, it.frame_num = seed0, frame0
     return {
         "tol": tol,
@@ -207,8 +236,8 @@ def time_to_rmse(it, ctx, ms_per_frame: float, gt_frames: int, gt_seed: int = 0x
                        "at_max_checked": {"frames": int(ks[-1]), "rmse": float(pts[-1][1])}},
         "standard_rmse_rel": {"definition": "sqrt(S/(3N)) / mean(ground truth)", "frames": std_k,
                               "seconds": None if std_k is None else round(std_k * ms_per_frame / 1e3, 3),
-                              "extrapolated": extrapolated, "sigma2_per_frame": sigma2,
-                              "at_max_checked": {"frames": int(ks[-1]), "rmse_rel": float(pts[-1][2] / mean)}},
+                              "at_max_checked": {"frames": int(ks[-1]), "rmse_rel": float(pts[-1][2] / mean)},
+                              "tail_fit": fit},
     }
 
 
@@ -356,7 +385,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--integrator", choices=["path", "bdpt", "vcm", "adjoint"], default="path")
+    ap.add_argument("--integrator", choices=["path", "bdpt", "vcm", "sppm", "adjoint"], default="path")
+    ap.add_argument("--mis", choices=["balance", "power"], default="power",
+                    help="BDPT MIS heuristic (the caustics workload specifies the power heuristic)")
     ap.add_argument("--width", type=int, default=0, help="default 1920 (path) / 1024 (bdpt, vcm)")
     ap.add_argument("--height", type=int, default=0, help="default 1080 (path) / 1024 (bdpt, vcm)")
     ap.add_argument("--estimator", choices=["corrected", "reference"], default="corrected",
@@ -382,27 +413,36 @@ def main() -> int:
     if args.integrator == "adjoint":
         return bench_adjoint(args)
 
+    import numpy as np
+
     import lumen_amd as la
 
     bdpt = args.integrator == "bdpt"
     vcm = args.integrator == "vcm"
-    W = args.width or (1024 if (bdpt or vcm) else 1920)
-    H = args.height or (1024 if (bdpt or vcm) else 1080)
-    kind = "caustics" if (bdpt or vcm) else "terrain"
+    sppm = args.integrator == "sppm"
+    caustics = bdpt or vcm or sppm
+    W = args.width or (1024 if caustics else 1920)
+    H = args.height or (1024 if caustics else 1080)
+    kind = "caustics" if caustics else "terrain"
     log(f"generating {kind} scene")
     t0 = time.time()
-    scene = la.Scene.procedural(kind, seed=7 if (bdpt or vcm) else 1)
+    scene = la.Scene.procedural(kind, seed=7 if caustics else 1)
     log(f"scene: {scene.num_triangles} triangles in {time.time() - t0:.1f}s")
     ctx = la.Context(0)
     n_cu, arch = ctx.device_info()
     vcm_kw = dict(radius_factor=1.0, alpha=0.75, use_vm=True, corrected=args.estimator == "corrected")
+    diag = float(np.linalg.norm(np.array(scene.desc.scene_max[:]) - np.array(scene.desc.scene_min[:])))
+    sppm_r0 = 0.005 * diag  # 0.5% of the scene diagonal, the VCM workload's r_0
     if bdpt:
-        it = la.BDPT(scene, ctx, W, H, path_length=8)
+        it = la.BDPT(scene, ctx, W, H, path_length=8, mis=args.mis)
         it.seed = 7
     elif vcm:
         # r_0 = 0.5% of the diagonal = 1% of the bounding radius (half diagonal)
         it = la.VCM(scene, ctx, W, H, path_length=8, radius_factor=1.0, alpha=0.75, enable_vm=True,
                     estimator=args.estimator)
+        it.seed = 7
+    elif sppm:
+        it = la.SPPM(scene, ctx, W, H, path_length=8, base_radius=sppm_r0)
         it.seed = 7
     else:
         it = la.Path(scene, ctx, W, H, strategy=args.strategy)
@@ -413,7 +453,7 @@ def main() -> int:
     sky = tuple(scene.integrator_cfg().sky_col)
     if bdpt and args.steps == 32 and "--steps" not in sys.argv:
         args.steps = 64  # the caustics workload is 64 spp
-    if vcm and args.steps == 32 and "--steps" not in sys.argv:
+    if (vcm or sppm) and args.steps == 32 and "--steps" not in sys.argv:
         args.steps = 100  # the VCM workload is 100 iterations
 
     for _ in range(args.warmup):
@@ -422,41 +462,105 @@ def main() -> int:
     ctx.synchronize()
     first = it.frame_num
 
-    # timed region: exactly K steps, synchronised on both sides
-    kernel_ms = []
+    # timed region: exactly K steps, synchronised on both sides; HIP events on
+    # the context stream time the whole frame and each of its phases
+    it.set_timing(True)
+    kernel_ms, phase_ms = [], []
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         it.render(spp=1)
-        kernel_ms.append(it.stats().kernel_ms)  # HIP events on the context stream
+        st = it.stats()
+        kernel_ms.append(st.kernel_ms)
+        phase_ms.append(list(st.phase_ms[:st.num_phases]))
         it.update()
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    it.set_timing(False)
     log(f"timed {args.steps} frames in {wall:.3f}s")
 
-    # untimed counting pass over the same frame indices (ray and BVH work counts
-    # are a function of the frame index only)
+    # untimed counting pass over the same frame indices (ray, BVH and photon
+    # work counts are a function of the frame index only; SPPM's per-pixel
+    # state is replayed from frame 0 to reach the same frames)
     it.set_counting(True)
-    rays = nodes = tris = closest = 0
+    c = dict(rays=0, closest=0, nodes=0, tris=0, snodes=0, stris=0, tested=0, gathered=0, gbytes=0)
+    if sppm:
+        it.frame_num = 0
+        for _ in range(first):
+            it.render(spp=1)
+            it.update()
     for k in range(args.steps):
         it.frame_num = first + k
         it.render(spp=1)
         s = it.stats()
-        rays += s.closest_rays + s.shadow_rays
-        closest += s.closest_rays
-        nodes += s.nodes_visited
-        tris += s.tris_tested
+        c["rays"] += s.closest_rays + s.shadow_rays
+        c["closest"] += s.closest_rays
+        c["nodes"] += s.nodes_visited
+        c["tris"] += s.tris_tested
+        c["snodes"] += s.shadow_nodes_visited
+        c["stris"] += s.shadow_tris_tested
+        c["tested"] += s.photons_tested
+        c["gathered"] += s.photons_gathered
+        c["gbytes"] += s.gather_bytes
     it.set_counting(False)
+    it.frame_num = first + args.steps
 
+    K = args.steps
+    rays = c["rays"]
     mrays = rays / wall / 1e6
     avg_ms = sum(kernel_ms) / len(kernel_ms)
-    bytes_per_launch = (nodes * 128 + tris * 48) / args.steps
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    workload = (f"caustics-sphere20k-{W}x{H}-bdpt-d8" if bdpt else
-                f"caustics-sphere20k-{W}x{H}-vcm-{args.estimator}-d8" if vcm else
-                "terrain-1M+2000-boxes-1920x1080-path-d12")
+    nph = min(len(p) for p in phase_ms)
+    ph_avg = [sum(p[i] for p in phase_ms) / K for i in range(nph)]
+    bvh_bytes = (c["nodes"] * 128 + c["tris"] * 48) / K
+    occl_bytes = (c["snodes"] * 128 + c["stris"] * 48) / K
 
-    gt_frames = args.gt_frames if args.gt_frames >= 0 else (128 if vcm else 1024)
+    def line(kernel, bytes_per_launch, ms, extra=None):
+        ach = bytes_per_launch / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        d = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "kernel_ms_avg": round(ms, 4),
+             "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        if extra:
+            d.update(extra)
+        return d
+
+    if bdpt:
+        workload = f"caustics-sphere20k-{W}x{H}-bdpt-{args.mis}-d8"
+        out_integrator = f"bdpt-{args.mis}"
+        # the walk traces the closest-hit rays, the connect pass the occlusion rays
+        roof = line("k_bdpt_walk", bvh_bytes - occl_bytes, ph_avg[0],
+                    {"other_kernels": {"k_bdpt_connect": line("k_bdpt_connect", occl_bytes, ph_avg[1]),
+                                       "k_bdpt_resolve_ms": round(ph_avg[-1], 4)}})
+    elif vcm and args.estimator == "corrected":
+        workload = f"caustics-sphere20k-{W}x{H}-vcm-corrected-d8"
+        out_integrator = "vcm-corrected"
+        roof = line("k_vcm_merge", c["gbytes"] / K, ph_avg[2],
+                    {"photons_tested_per_launch": c["tested"] // K, "photons_merged_per_launch": c["gathered"] // K,
+                     "other_kernels": {
+                         "light_pass_and_grid_plus_eye_pass": line("k_vcm_light + grid + k_vcm_eye", bvh_bytes,
+                                                                   ph_avg[0] + ph_avg[1]),
+                         "phase_ms": [round(v, 4) for v in ph_avg]}})
+    elif vcm:
+        workload = f"caustics-sphere20k-{W}x{H}-vcm-reference-d8"
+        out_integrator = "vcm-reference"
+        roof = line("one frame's launch sequence (light pass, photon grid, eye pass)", bvh_bytes, avg_ms,
+                    {"phase_ms": [round(v, 4) for v in ph_avg]})
+    elif sppm:
+        workload = f"caustics-sphere20k-{W}x{H}-sppm-d8"
+        out_integrator = "sppm"
+        roof = line("k_sppm_eye + k_sppm_light (the traversal kernels)", bvh_bytes, ph_avg[0] + ph_avg[2],
+                    {"other_kernels": {"k_sppm_gather": line("k_sppm_gather", c["gbytes"] / K, ph_avg[3]),
+                                       "phase_ms": [round(v, 4) for v in ph_avg]},
+                     "photons_tested_per_launch": c["tested"] // K,
+                     "photons_gathered_per_launch": c["gathered"] // K})
+    else:
+        workload = "terrain-1M+2000-boxes-1920x1080-path-d12"
+        out_integrator = f"path-{args.strategy}"
+        roof = line("k_path_mega" if args.strategy == "megakernel" else
+                    "k_path_regen (one launch per frame)" if args.strategy == "persistent" else
+                    "one frame's launch sequence (trace + shade per depth)", bvh_bytes, avg_ms)
+    roof["traffic"] = traffic_from_profile(workload, out_integrator)
+
+    gt_frames = args.gt_frames if args.gt_frames >= 0 else (128 if (vcm or sppm) else 1024)
     rmse = None
     if gt_frames > 0:
         log(f"time to RMSE: ground truth of {gt_frames} frames")
@@ -466,16 +570,17 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         if bdpt:
-            cpu = cpu_baseline_bdpt(scene, it.camera, W, H, depth, first, args.cpu_seconds)
+            cpu = cpu_baseline_bdpt(scene, it.camera, W, H, depth, first, args.cpu_seconds, args.mis == "power")
         elif vcm:
             cpu = cpu_baseline_vcm(scene, it.camera, W, H, depth, first, args.cpu_seconds, vcm_kw)
+        elif sppm:
+            cpu = cpu_baseline_sppm(scene, it.camera, W, H, depth, args.cpu_seconds, sppm_r0)
         else:
             cpu = cpu_baseline(scene, it.camera, W, H, depth, sky, first, args.cpu_seconds)
 
-    out_integrator = "bdpt" if bdpt else f"vcm-{args.estimator}" if vcm else f"path-{args.strategy}"
     out = {
         "metric": ("Mrays/s (BDPT, 1 GPU)" if bdpt else "Mrays/s (VCM, 1 GPU)" if vcm else
-                   "Mrays/s (Path, 1080p, 1 GPU)"),
+                   "Mrays/s (SPPM, 1 GPU)" if sppm else "Mrays/s (Path, 1080p, 1 GPU)"),
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -496,24 +601,10 @@ def main() -> int:
             "max_depth": depth,
             "triangles": scene.num_triangles,
             "rays_per_step": rays // args.steps,
-            "closest_rays_per_step": closest // args.steps,
-            "shadow_rays_per_step": (rays - closest) // args.steps,
+            "closest_rays_per_step": c["closest"] // args.steps,
+            "shadow_rays_per_step": (rays - c["closest"]) // args.steps,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(workload, out_integrator),
-            "kernel": ("one frame's launch sequence (walk + connect per pixel chunk, resolve)" if bdpt else
-                       "one frame's launch sequence (light pass, photon grid, eye pass)" if vcm else
-                       "k_path_mega" if args.strategy == "megakernel" else
-                       "k_path_regen (one launch per frame)" if args.strategy == "persistent" else
-                       "one frame's launch sequence (trace + shade per depth)"),
-            "kernel_ms_avg": round(avg_ms, 3),
-            "algorithmic_bytes_per_launch": int(bytes_per_launch),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
         "time_to_rmse": rmse,
         "bvh_build_ms": round(bst.build_ms, 2),

@@ -304,7 +304,8 @@ lt_status lt_stream_copy(lt_context* ctx, void* dst, const void* src, size_t byt
 typedef enum lt_integrator_kind {
     LT_INTEGRATOR_PATH = 0,
     LT_INTEGRATOR_BDPT = 1,
-    LT_INTEGRATOR_VCM = 2
+    LT_INTEGRATOR_VCM = 2,
+    LT_INTEGRATOR_SPPM = 3
 } lt_integrator_kind;
 
 /* Path execution strategies. */
@@ -375,6 +376,24 @@ typedef struct lt_vcm_config {
     uint32_t reserved[6];
 } lt_vcm_config;
 
+/* SPPM integrator config: SceneConfig/SPPMConfig (src/RayTracer/SceneConfig.h:33-36)
+ * plus the PCSPPM fields set by SPPM::render (src/RayTracer/SPPM.cpp:75-104).
+ * Eye rays go through pixel centres (sppm_eye.rgen:38-40); one light path per
+ * pixel slot per frame; the per-pixel state (recorded point, radius, photon
+ * count N) persists from frame 0 on.  The reference's photon table keeps one
+ * photon per hash bucket written by racing threads (sppm_light.rgen:85-93);
+ * here the bucket keeps the photon of the largest (light path, vertex), so
+ * images are deterministic. */
+typedef struct lt_sppm_config {
+    uint32_t width;
+    uint32_t height;
+    int32_t max_depth;   /* path_length, 1..16 (default 6)                     */
+    float sky_col[3];
+    float base_radius;   /* initial gather radius (default 0.03; SPPM.cpp:86-88
+                            raises it to at least 1e-7)                          */
+    uint32_t reserved[8];
+} lt_sppm_config;
+
 /* Per-frame parameters (SceneUBO + push-constant frame_num). */
 typedef struct lt_frame_params {
     lt_camera camera;
@@ -392,13 +411,31 @@ typedef struct lt_render_stats {
     uint64_t tris_tested;    /* triangle records tested (48 B each)   */
     float kernel_ms;         /* device time of the render             */
     uint32_t frames;
+    /* Photon gathers (VCM corrected estimator's merge pass, SPPM gather), with
+     * counting on: photons whose position was tested against the radius,
+     * photons gathered, and the gather pass's algorithmic bytes (records of
+     * the queries, cell ranges, photon positions and photon records read). */
+    uint64_t photons_tested;
+    uint64_t photons_gathered;
+    uint64_t gather_bytes;
+    /* The part of nodes_visited / tris_tested that occlusion queries took. */
+    uint64_t shadow_nodes_visited;
+    uint64_t shadow_tris_tested;
+    /* With lt_render_set_timing on: device time of each phase of the first
+     * frame of the last lt_render.  Phases: Path persistent / megakernel: 1
+     * (the frame); Path wavefront: 1; BDPT: walk and connect of each pixel
+     * chunk (one chunk at 1024x1024, depth 8), then the splat resolve; VCM: 4 (light pass + photon grid, eye pass, merge pass,
+     * resolve; the last two 0 without the corrected estimator's merging);
+     * SPPM: 4 (eye pass, grid bounds, light pass, gather). */
+    uint32_t num_phases;
+    float phase_ms[8];
 } lt_render_stats;
 
 /* Replaces RayTracer::create_integrator + Integrator::init
  * (src/RayTracer/RayTracer.cpp:244-282, Integrator.cpp:8-50).  cfg points to
  * the config struct of `kind` (lt_path_config for LT_INTEGRATOR_PATH,
- * lt_bdpt_config for LT_INTEGRATOR_BDPT, lt_vcm_config for LT_INTEGRATOR_VCM)
- * and cfg_size must equal its size. */
+ * lt_bdpt_config for LT_INTEGRATOR_BDPT, lt_vcm_config for LT_INTEGRATOR_VCM,
+ * lt_sppm_config for LT_INTEGRATOR_SPPM) and cfg_size must equal its size. */
 lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const void* cfg, size_t cfg_size,
                                lt_integrator** out);
 lt_status lt_integrator_destroy(lt_integrator* it);
@@ -411,7 +448,11 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* params);
 lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* stats);
 /* Enables per-ray counting in later lt_render calls (slower; for Mrays/s). */
 lt_status lt_render_set_counting(lt_integrator* it, int enable);
-/* Frames from a captured hipGraph (Path and BDPT; VCM: LT_ERR_UNSUPPORTED).
+/* Enables per-phase device timing of later lt_render calls (HIP events on the
+ * context stream between the phases of the first frame; see lt_render_stats).
+ * No reference counterpart.  Not with frame graphs. */
+lt_status lt_render_set_timing(lt_integrator* it, int enable);
+/* Frames from a captured hipGraph (Path and BDPT; VCM and SPPM: LT_ERR_UNSUPPORTED).
  * No reference counterpart.  The first lt_render after enabling, or after a
  * change of camera, seed, counting, scene, BVH or launch shapes, captures one
  * frame's launch sequence; each frame is then one graph launch whose kernels
@@ -519,13 +560,15 @@ lt_status lt_scene_camera(const lt_host_scene* s, uint32_t width, uint32_t heigh
  * LumenScene.cpp:487-491). */
 lt_status lt_scene_set_camera(lt_host_scene* s, float fov, const float pos[3], const float dir[3]);
 /* Integrator block of the scene JSON (LumenScene.cpp:237-269): type
- * (lt_integrator_kind), path_length, sky_col, VCM enable_vm / radius_factor. */
+ * (lt_integrator_kind), path_length, sky_col, VCM enable_vm / radius_factor,
+ * SPPM base_radius. */
 typedef struct lt_scene_integrator_cfg {
     int32_t kind;
     int32_t path_length;
     float sky_col[3];
     int32_t enable_vm;
     float radius_factor;
+    float base_radius;
 } lt_scene_integrator_cfg;
 lt_status lt_scene_integrator(const lt_host_scene* s, lt_scene_integrator_cfg* out);
 

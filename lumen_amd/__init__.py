@@ -13,11 +13,11 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRATOR_VCM, LT_PATH_MEGAKERNEL,
-                   LT_PATH_PERSISTENT, LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS, LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL,
-                   LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
+from ._abi import (HIT_DTYPE, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_PATH, LT_INTEGRATOR_SPPM, LT_INTEGRATOR_VCM,
+                   LT_PATH_MEGAKERNEL, LT_PATH_PERSISTENT, LT_PATH_WAVEFRONT, LT_PTR_DEVICE, LT_SCENE_CAUSTICS,
+                   LT_SCENE_CORNELL, LT_SCENE_DIFF_WALL, LT_SCENE_TERRAIN, RAY_DTYPE, LtError, check, lib)
 
-__all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "VCM", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
+__all__ = ["Scene", "Context", "Integrator", "Path", "BDPT", "VCM", "SPPM", "create_integrator", "RAY_DTYPE", "HIT_DTYPE", "LtError",
            "lib", "make_rays", "save_exr", "load_exr"]
 
 _KINDS = {"cornell": LT_SCENE_CORNELL, "terrain": LT_SCENE_TERRAIN, "caustics": LT_SCENE_CAUSTICS,
@@ -299,6 +299,10 @@ class Integrator:
     def set_counting(self, on: bool) -> None:
         check(lib().lt_render_set_counting(self._it, int(on)), self.ctx.handle, "lt_render_set_counting")
 
+    def set_timing(self, on: bool) -> None:
+        """Per-phase device timing of later one-frame renders (stats().phase_ms)."""
+        check(lib().lt_render_set_timing(self._it, int(on)), self.ctx.handle, "lt_render_set_timing")
+
     def stats(self) -> _abi.lt_render_stats:
         s = _abi.lt_render_stats()
         check(lib().lt_render_stats_get(self._it, C.byref(s)), self.ctx.handle, "lt_render_stats_get")
@@ -409,6 +413,33 @@ class VCM(Integrator):
         return c
 
 
+class SPPM(Integrator):
+    """Stochastic progressive photon mapping (src/RayTracer/SPPM.{h,cpp}):
+    per frame an eye pass through pixel centres that records the first
+    diffuse surface, a light pass whose photons fill a one-photon-per-bucket
+    hash grid sized by the largest gather radius, and a gather with the
+    progressive radius update (alpha = 2/3).  The per-pixel state persists
+    from frame 0 on."""
+
+    kind = LT_INTEGRATOR_SPPM
+
+    def __init__(self, lumen_scene, ctx, width=1920, height=1080, path_length=None, sky_col=None,
+                 base_radius=None):
+        super().__init__(lumen_scene, ctx, width, height)
+        icfg = lumen_scene.integrator_cfg()
+        self.path_length = path_length if path_length is not None else (icfg.path_length or 6)
+        self.sky_col = sky_col if sky_col is not None else tuple(icfg.sky_col)
+        self.base_radius = base_radius if base_radius is not None else (icfg.base_radius or 0.03)
+
+    def _config(self):
+        c = _abi.lt_sppm_config()
+        c.width, c.height = self.width, self.height
+        c.max_depth = self.path_length
+        c.sky_col[:] = list(self.sky_col)
+        c.base_radius = self.base_radius
+        return c
+
+
 def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 1920, height: int = 1080,
                       **kw) -> Integrator:
     """RayTracer::create_integrator (RayTracer.cpp:244-282), keyed by the
@@ -420,6 +451,8 @@ def create_integrator(name: str, lumen_scene: Scene, ctx: Context, width: int = 
         return BDPT(lumen_scene, ctx, width, height, **kw)
     if n == "vcm":
         return VCM(lumen_scene, ctx, width, height, **kw)
+    if n == "sppm":
+        return SPPM(lumen_scene, ctx, width, height, **kw)
     raise LtError(6, f"integrator '{name}' is not built")
 
 

@@ -28,7 +28,7 @@ LT_LIGHT_FINITE, LT_LIGHT_DELTA = 1 << 4, 1 << 5
 LT_PTR_DEVICE = 1
 LT_NEE_REFERENCE = 0
 LT_NEE_CORRECTED = 1
-LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM = 0, 1, 2
+LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM, LT_INTEGRATOR_SPPM = 0, 1, 2, 3
 LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT, LT_PATH_PERSISTENT = 0, 1, 2
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
@@ -107,6 +107,11 @@ class lt_vcm_config(C.Structure):
                 ("estimator", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
+class lt_sppm_config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("sky_col", C.c_float * 3),
+                ("base_radius", C.c_float), ("reserved", C.c_uint32 * 8)]
+
+
 class lt_adjoint_config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32), ("spp", C.c_uint32),
                 ("variants", C.c_uint32), ("albedo_texture", C.c_int32), ("roughness_texture", C.c_int32),
@@ -120,7 +125,10 @@ class lt_frame_params(C.Structure):
 
 class lt_render_stats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
-                ("tris_tested", C.c_uint64), ("kernel_ms", C.c_float), ("frames", C.c_uint32)]
+                ("tris_tested", C.c_uint64), ("kernel_ms", C.c_float), ("frames", C.c_uint32),
+                ("photons_tested", C.c_uint64), ("photons_gathered", C.c_uint64), ("gather_bytes", C.c_uint64),
+                ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64),
+                ("num_phases", C.c_uint32), ("phase_ms", C.c_float * 8)]
 
 
 class lt_procedural_params(C.Structure):
@@ -137,7 +145,7 @@ class lt_tuning(C.Structure):
 
 class lt_scene_integrator_cfg(C.Structure):
     _fields_ = [("kind", C.c_int32), ("path_length", C.c_int32), ("sky_col", C.c_float * 3),
-                ("enable_vm", C.c_int32), ("radius_factor", C.c_float)]
+                ("enable_vm", C.c_int32), ("radius_factor", C.c_float), ("base_radius", C.c_float)]
 
 
 # numpy views of the ray / hit records
@@ -172,6 +180,7 @@ _PROTOS = {
     "lt_integrator_destroy": (C.c_int, [P]),
     "lt_render": (C.c_int, [P, C.POINTER(lt_frame_params)]),
     "lt_render_stats_get": (C.c_int, [P, C.POINTER(lt_render_stats)]),
+    "lt_render_set_timing": (C.c_int, [P, C.c_int]),
     "lt_render_set_counting": (C.c_int, [P, C.c_int]),
     "lt_render_set_graph": (C.c_int, [P, C.c_int]),
     "lt_accum_reset": (C.c_int, [P]),

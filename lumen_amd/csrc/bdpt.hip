@@ -23,6 +23,7 @@
 
 #include "bdpt.h"
 #include "dev_scene.h"
+#include "dev_count.h"
 #include "dev_trace.h"
 
 using namespace ltd;
@@ -45,10 +46,6 @@ enum : int {
     F_PREV = 15,
     F_PRE = 16,   // MIS prefix P_k / Q_k
     kVtxFields = 17
-};
-
-struct Ctr {
-    uint32_t closest, shadow, nodes, tris;
 };
 
 struct Vtx {
@@ -123,7 +120,7 @@ template <bool COUNT>
 LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk, Ctr& c) {
     if (COUNT) c.shadow++;
     HitRec h;
-    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.nodes, c.tris);
+    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.snodes, c.stris);
 }
 
 LT_DEV f3 eval_f(const SceneView& sv, const Vtx& v, const Mat& m, f3 wo, f3 wi) {
@@ -660,22 +657,6 @@ LT_DEV f3 bdpt_connect(const SceneView& sv, const BdptLaunch& L, const VStore& v
     return col;
 }
 
-LT_DEV void flush_counts(const Ctr& c, uint32_t lane, unsigned long long* counts) {
-    unsigned long long a = c.closest, b = c.shadow, n = c.nodes, tt = c.tris;
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o);
-        b += __shfl_xor(b, o);
-        n += __shfl_xor(n, o);
-        tt += __shfl_xor(tt, o);
-    }
-    if (lane == 0) {
-        atomicAdd(&counts[0], a);
-        atomicAdd(&counts[1], b);
-        atomicAdd(&counts[2], n);
-        atomicAdd(&counts[3], tt);
-    }
-}
-
 // Both kernels are persistent: each wave pulls 8x8 pixel tiles of the chunk
 // [tile0, tile0 + n_tiles) from `work`; a pixel's slot in the vertex and
 // record buffers is its position in the chunk.
@@ -753,7 +734,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
             }
         }
     }
-    if (COUNT) flush_counts(c, lane, counts);
+    if (COUNT) flush_counts(c, counts);
 }
 
 #ifndef LT_BDPT_CONN_WAVES
@@ -785,7 +766,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         const f3 col = bdpt_connect<COUNT>(sv, L, vs, hdr, x, y, splat, stk, c);
         fcol[(size_t)y * L.width + x] = make_float4(col.x, col.y, col.z, 0.0f);
     }
-    if (COUNT) flush_counts(c, lane, counts);
+    if (COUNT) flush_counts(c, counts);
 }
 
 // bdpt.rgen:77-90: camera estimate + splats, NaN guard, running mean
@@ -883,7 +864,9 @@ lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L0, uin
     if (st != LT_OK) return st;
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
-    if (count) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, (size_t)8, 0u);
+    if (count)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)d_counts, (size_t)(2 * kCountSlots),
+                           0u);
     const uint32_t n = (uint32_t)b.pixels;
     const dim3 wgrid(walk_blocks), cgrid(conn_blocks), block(kTraceBlock);
     for (uint32_t f = 0; f < frames; f++) {
@@ -893,19 +876,23 @@ lt_status launch_bdpt(lt_context* ctx, BdptBuffers& b, const BdptLaunch& L0, uin
         L.regen_min = ctx->tuning.regen_min ? ctx->tuning.regen_min : 4u;
         for (uint32_t t0 = 0; t0 < tiles; t0 += b.chunk_tiles) {
             const uint32_t nt = tiles - t0 < b.chunk_tiles ? tiles - t0 : b.chunk_tiles;
+            if (f == 0) phase_mark(ctx);
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, (size_t)2, 0u);
-            if (count) {
+            if (count)
                 hipLaunchKernelGGL((k_bdpt_walk<true>), wgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
                                    t0, nt, ctx->counters, ctx->spill, cnt);
-                hipLaunchKernelGGL((k_bdpt_connect<true>), cgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
-                                   b.slots, t0, nt, b.fcol, b.splat, ctx->counters + 1, ctx->spill, cnt);
-            } else {
+            else
                 hipLaunchKernelGGL((k_bdpt_walk<false>), wgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr, b.slots,
                                    t0, nt, ctx->counters, ctx->spill, cnt);
+            if (f == 0) phase_mark(ctx);
+            if (count)
+                hipLaunchKernelGGL((k_bdpt_connect<true>), cgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
+                                   b.slots, t0, nt, b.fcol, b.splat, ctx->counters + 1, ctx->spill, cnt);
+            else
                 hipLaunchKernelGGL((k_bdpt_connect<false>), cgrid, block, 0, ctx->stream, sv, L, b.verts, b.hdr,
                                    b.slots, t0, nt, b.fcol, b.splat, ctx->counters + 1, ctx->spill, cnt);
-            }
         }
+        if (f == 0) phase_mark(ctx);
         hipLaunchKernelGGL(k_bdpt_resolve, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, b.fcol, b.splat, accum,
                            n, frame_num + f, L0.frame_dev);
     }

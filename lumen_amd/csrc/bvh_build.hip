@@ -5,7 +5,7 @@
 // src/Framework/AccelerationStructure.cpp:171-315).  Stages:
 //   1. k_make_tris     : 48-B triangle records + centroid bounds (atomics)
 //   2. k_morton        : 30-bit Morton codes of the centroids
-//   3. radix sort      : (morton, triangle) pairs, hipcub/rocPRIM onesweep
+//   3. radix sort      : (morton, triangle) pairs, rocPRIM radix sort (bits 0..29)
 //   4. k_karras        : binary radix tree (Karras 2012), one thread per node
 //   5. k_refit         : bottom-up AABBs with arrival counters
 //   6. k_collapse      : top-down BVH8 collapse, one thread per BVH8 node and
@@ -14,7 +14,7 @@
 //                        leaves; 16-bit quantised, 128-B nodes.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <cmath>
 #include <vector>
 
@@ -450,9 +450,9 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
         BCHK(hipGetLastError());
         hipLaunchKernelGGL(k_morton, dim3(G), dim3(B), 0, st, tris_unsorted, n, cb, keys, vals);
         BCHK(hipGetLastError());
-        BCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, st));
+        BCHK(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, n, 0u, 30u, st));
         BCHK(hipMalloc(&sort_tmp, sort_bytes));
-        BCHK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, (int)n, 0, 30, st));
+        BCHK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, 30u, st));
         if (n > 1) {
             hipLaunchKernelGGL(k_karras, dim3((n - 1 + B - 1) / B), dim3(B), 0, st, keys2, (int)n, left, right, pint,
                                pleaf, rlo, rhi);

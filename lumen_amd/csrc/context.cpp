@@ -103,7 +103,7 @@ lt_status lt_context_create(int device, lt_context** out) {
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc(&ctx->counters, 64 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&ctx->count_buf, 4 * sizeof(uint64_t)) != hipSuccess) {
+        hipMalloc(&ctx->count_buf, kCountSlots * sizeof(uint64_t)) != hipSuccess) {
         lt_context_destroy(ctx);
         return LT_ERR_HIP;
     }
@@ -126,6 +126,8 @@ lt_status lt_context_destroy(lt_context* ctx) {
     void* ptrs[] = {ctx->spill, ctx->counters, ctx->ray_buf, ctx->hit_buf, ctx->count_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t& e : ctx->phase_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -370,3 +372,8 @@ lt_status lt_synchronize(lt_context* ctx) {
 }
 
 }  // extern "C"
+
+void phase_mark(lt_context* ctx) {
+    if (!ctx->phase_on || ctx->phase_n > kMaxPhases) return;
+    if (hipEventRecord(ctx->phase_ev[ctx->phase_n], ctx->stream) == hipSuccess) ctx->phase_n++;
+}

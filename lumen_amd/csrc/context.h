@@ -85,7 +85,12 @@ struct lt_context {
     size_t ray_buf_bytes = 0;
     void* hit_buf = nullptr;
     size_t hit_buf_bytes = 0;
-    uint64_t* count_buf = nullptr;  // [closest, shadow, nodes, tris]
+    uint64_t* count_buf = nullptr;  // kCountSlots counters of a counting render
+    // device timing of the phases of a render (lt_render_set_timing): the
+    // launch code marks each phase's start and the frame's end
+    int phase_on = 0;
+    int phase_n = 0;
+    hipEvent_t phase_ev[9] = {};
     // device address of the no-allocation entry point's scratch array (shared
     // by every context on this device; not owned)
     uint32_t* noalloc_scratch = nullptr;
@@ -110,6 +115,15 @@ SceneView make_scene_view(const lt_context* ctx);
 // persistent grid: number of resident threads the spill buffer is sized for
 lt_status ensure_spill(lt_context* ctx, size_t threads);
 constexpr int kTraceBlock = 128;
+// counters of a counting render, u64 each: closest rays, shadow rays, BVH8
+// nodes popped, triangles tested, photons tested against a gather radius,
+// photons gathered, algorithmic bytes of the gather pass, the nodes and the
+// triangles of occlusion queries (included in slots 2 and 3), spare
+constexpr int kCountSlots = 10;
+constexpr int kMaxPhases = 8;
+// records the next phase boundary of the first frame of a render when phase
+// timing is on (at most kMaxPhases phases)
+void phase_mark(lt_context* ctx);
 // workgroups of kTraceBlock threads of `kernel` that one CU holds at once
 // (its register and LDS use), at least 1; `fixed` (a tuning value) wins if set
 uint32_t resident_blocks(const void* kernel, uint32_t fixed);

@@ -3,6 +3,7 @@
 // points (lt_scene_*).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -12,6 +13,7 @@
 #include "path.h"
 #include "path_wave.h"
 #include "scene.h"
+#include "sppm.h"
 #include "vcm.h"
 
 lt_status launch_sq_error(lt_context* ctx, const float4* a, const float4* b, size_t n, double* d_out);
@@ -23,11 +25,15 @@ struct lt_integrator {
     lt_path_config path{};
     lt_bdpt_config bdpt{};
     lt_vcm_config vcm{};
+    lt_sppm_config sppm{};
     float4* accum = nullptr;  // output_tex, RGBA32F, row-major
     WaveBuffers wave;         // wavefront state (LT_PATH_WAVEFRONT)
     BdptBuffers bb;           // BDPT subpaths, camera estimates and splats
     VcmBuffers vb;            // VCM light vertices, photon grid and splats
+    SppmBuffers sb;           // SPPM per-pixel state, photons and photon table
     int counting = 0;
+    int timing = 0;   // per-phase events (lt_render_set_timing)
+    int phases = 0;   // phase marks recorded by the last render
     lt_render_stats last{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool rendered = false;
@@ -96,6 +102,18 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
             return lt_fail(ctx, LT_ERR_UNSUPPORTED, "VCM image too large for 32-bit photon references");
         W = vc->width;
         H = vc->height;
+    } else if (kind == LT_INTEGRATOR_SPPM) {
+        if (cfg_size != sizeof(lt_sppm_config)) return lt_fail(ctx, LT_ERR_INVALID_ARG, "cfg_size mismatch");
+        const lt_sppm_config* sc = (const lt_sppm_config*)cfg;
+        if (sc->max_depth < 1 || sc->max_depth > kSppmMaxDepth)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "SPPM path length must be in 1..16");
+        if (!(sc->base_radius >= 0.0f) || !std::isfinite(sc->base_radius))
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "SPPM base_radius must be finite and >= 0");
+        if ((uint64_t)sc->width * sc->height * 10u >= (1ull << 31) ||
+            (uint64_t)sc->width * sc->height * (uint64_t)sc->max_depth >= (1ull << 32) - 1u)
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "SPPM image too large for 32-bit photon references");
+        W = sc->width;
+        H = sc->height;
     } else {
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown integrator kind");
     }
@@ -110,8 +128,10 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
         it->path = *(const lt_path_config*)cfg;
     else if (kind == LT_INTEGRATOR_BDPT)
         it->bdpt = *(const lt_bdpt_config*)cfg;
-    else
+    else if (kind == LT_INTEGRATOR_VCM)
         it->vcm = *(const lt_vcm_config*)cfg;
+    else
+        it->sppm = *(const lt_sppm_config*)cfg;
     (void)hipSetDevice(ctx->device);
     size_t bytes = (size_t)W * H * sizeof(float4);
     hipError_t e = hipMalloc(&it->accum, bytes);
@@ -123,9 +143,10 @@ lt_status lt_integrator_create(lt_context* ctx, lt_integrator_kind kind, const v
         lt_integrator_destroy(it);
         return st;
     }
-    if (kind == LT_INTEGRATOR_BDPT || kind == LT_INTEGRATOR_VCM) {
-        lt_status st = kind == LT_INTEGRATOR_BDPT ? bdpt_alloc(ctx, it->bb, W, H, it->bdpt.max_depth)
-                                                  : vcm_alloc(ctx, it->vb, W, H, it->vcm.max_depth);
+    if (kind == LT_INTEGRATOR_BDPT || kind == LT_INTEGRATOR_VCM || kind == LT_INTEGRATOR_SPPM) {
+        lt_status st = kind == LT_INTEGRATOR_BDPT  ? bdpt_alloc(ctx, it->bb, W, H, it->bdpt.max_depth)
+                       : kind == LT_INTEGRATOR_VCM ? vcm_alloc(ctx, it->vb, W, H, it->vcm.max_depth)
+                                                   : sppm_alloc(ctx, it->sb, W, H, it->sppm.max_depth);
         if (st != LT_OK) {
             lt_integrator_destroy(it);
             return st;
@@ -142,6 +163,7 @@ lt_status lt_integrator_destroy(lt_integrator* it) {
     wave_free(it->wave);
     bdpt_free(it->bb);
     vcm_free(it->vb);
+    sppm_free(it->sb);
     if (it->e0) (void)hipEventDestroy(it->e0);
     if (it->e1) (void)hipEventDestroy(it->e1);
     drop_graph(it);
@@ -184,6 +206,18 @@ static lt_status launch_frames(lt_integrator* it, const lt_frame_params* p, uint
         L.use_vm = it->vcm.enable_vm ? 1u : 0u;
         L.corrected = it->vcm.estimator == LT_VCM_CORRECTED ? 1u : 0u;
         return launch_vcm(ctx, it->vb, L, it->vcm, frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
+    }
+    if (it->kind == LT_INTEGRATOR_SPPM) {
+        SppmLaunch L{};
+        memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
+        memcpy(L.inv_projection, p->camera.inv_projection, sizeof(L.inv_projection));
+        L.width = it->width;
+        L.height = it->height;
+        L.n_px = it->width * it->height;
+        L.max_depth = it->sppm.max_depth;
+        L.base_radius = it->sppm.base_radius < 1e-7f ? 1e-7f : it->sppm.base_radius;  // SPPM.cpp:86-88
+        memcpy(L.sky_col, it->sppm.sky_col, 12);
+        return launch_sppm(ctx, it->sb, L, frame_num, spp, p->seed, it->accum, it->counting, ctx->count_buf);
     }
     PathLaunch L{};
     memcpy(L.inv_view, p->camera.inv_view, sizeof(L.inv_view));
@@ -245,6 +279,7 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
     const uint32_t spp = p->spp ? p->spp : 1u;
     lt_status st;
     if (it->graph_on) {
+        ctx->phase_on = 0;
         const bool stale = !it->gexec || memcmp(&it->graph_cam, &p->camera, sizeof(lt_camera)) != 0 ||
                            it->graph_seed != p->seed || it->graph_counting != it->counting ||
                            it->graph_gen != ctx->generation;
@@ -256,8 +291,17 @@ lt_status lt_render(lt_integrator* it, const lt_frame_params* p) {
         it->graph_word_known = true;
         it->graph_word = p->frame_num + spp;
     } else {
+        ctx->phase_on = it->timing;
+        ctx->phase_n = 0;
+        if (it->timing)
+            for (hipEvent_t& e : ctx->phase_ev)
+                if (!e) LT_HIP(ctx, hipEventCreate(&e));
         LT_HIP(ctx, hipEventRecord(it->e0, ctx->stream));
-        if ((st = launch_frames(it, p, p->frame_num, spp, nullptr)) != LT_OK) return st;
+        st = launch_frames(it, p, p->frame_num, spp, nullptr);
+        phase_mark(ctx);  // the end of the last phase (of a one-frame render)
+        it->phases = ctx->phase_n;
+        ctx->phase_on = 0;
+        if (st != LT_OK) return st;
     }
     LT_HIP(ctx, hipEventRecord(it->e1, ctx->stream));
     it->last.frames = spp;
@@ -269,6 +313,8 @@ lt_status lt_render_set_graph(lt_integrator* it, int enable) {
     if (!it) return LT_ERR_INVALID_ARG;
     if (enable && it->kind == LT_INTEGRATOR_VCM)
         return lt_fail(it->ctx, LT_ERR_UNSUPPORTED, "VCM frames set their radius and grid on the host");
+    if (enable && it->kind == LT_INTEGRATOR_SPPM)
+        return lt_fail(it->ctx, LT_ERR_UNSUPPORTED, "SPPM frames set their RNG word and state reset on the host");
     it->graph_on = enable ? 1 : 0;
     if (!enable) drop_graph(it);
     return LT_OK;
@@ -283,16 +329,36 @@ lt_status lt_render_stats_get(lt_integrator* it, lt_render_stats* s) {
     LT_HIP(ctx, hipEventElapsedTime(&ms, it->e0, it->e1));
     it->last.kernel_ms = ms;
     it->last.closest_rays = it->last.shadow_rays = it->last.nodes_visited = it->last.tris_tested = 0;
+    it->last.photons_tested = it->last.photons_gathered = it->last.gather_bytes = 0;
+    it->last.shadow_nodes_visited = it->last.shadow_tris_tested = 0;
     if (it->counting) {
-        uint64_t h[4];
-        LT_HIP(ctx, hipMemcpyAsync(h, ctx->count_buf, 32, hipMemcpyDeviceToHost, ctx->stream));
+        uint64_t h[kCountSlots];
+        LT_HIP(ctx, hipMemcpyAsync(h, ctx->count_buf, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
         LT_HIP(ctx, hipStreamSynchronize(ctx->stream));
         it->last.closest_rays = h[0];
         it->last.shadow_rays = h[1];
         it->last.nodes_visited = h[2];
         it->last.tris_tested = h[3];
+        it->last.photons_tested = h[4];
+        it->last.photons_gathered = h[5];
+        it->last.gather_bytes = h[6];
+        it->last.shadow_nodes_visited = h[7];
+        it->last.shadow_tris_tested = h[8];
+    }
+    it->last.num_phases = 0;
+    for (float& v : it->last.phase_ms) v = 0.0f;
+    if (it->timing && it->phases > 1 && it->last.frames == 1) {
+        it->last.num_phases = (uint32_t)(it->phases - 1);
+        for (int i = 0; i + 1 < it->phases; i++)
+            LT_HIP(ctx, hipEventElapsedTime(&it->last.phase_ms[i], ctx->phase_ev[i], ctx->phase_ev[i + 1]));
     }
     *s = it->last;
+    return LT_OK;
+}
+
+lt_status lt_render_set_timing(lt_integrator* it, int enable) {
+    if (!it) return LT_ERR_INVALID_ARG;
+    it->timing = enable ? 1 : 0;
     return LT_OK;
 }
 

@@ -25,7 +25,7 @@ struct PathLaunch {
 // Probability of the NEE light pick that sample_light_Li made (light `li`):
 // the reference's 1 / light_triangle_count (path.rgen:76-78), or
 // 1 / (lights x the picked light's triangles).
-LT_DEV inline float nee_pick_pdf(const SceneView& sv, uint32_t nee_pick, uint32_t li) {
+LT_DEV float nee_pick_pdf(const SceneView& sv, uint32_t nee_pick, uint32_t li) {
     if (nee_pick != LT_NEE_CORRECTED) return 1.0f / (float)sv.total_light_triangle_cnt;
     const lt_light& l = sv.lights[li];
     return 1.0f / ((float)sv.num_lights * ((l.light_flags & 7u) == LT_LIGHT_AREA ? (float)l.num_triangles : 1.0f));

@@ -438,7 +438,8 @@ lt_status launch_path_wavefront(lt_context* ctx, WaveBuffers& wb, const PathLaun
     const size_t sblocks = (size_t)ctx->num_cus * 8;
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
-    if (count) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, 4u);
+    if (count) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint32_t)kCountSlots);
+    phase_mark(ctx);
     const uint32_t depth_n = L.max_depth > 0 ? (uint32_t)L.max_depth : 1u;
     for (uint32_t f = 0; f < L.spp; f++) {
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, ctx->stream, wb.ctl);

@@ -438,17 +438,20 @@ lt_status load_json_scene(const char* path, lt_host_scene** out, std::string& er
     const JVal* integ = root.get("integrator");
     s->icfg.kind = -1;
     s->icfg.path_length = 6;
+    s->icfg.base_radius = 0.03f;
     if (integ) {
         const JVal* ty = integ->get("type");
         const std::string type = ty && ty->kind == JVal::STR ? ty->str : "";
         s->icfg.kind = type == "path" ? LT_INTEGRATOR_PATH
                        : type == "bdpt" ? LT_INTEGRATOR_BDPT
                        : type == "vcm"  ? LT_INTEGRATOR_VCM
+                       : type == "sppm" ? LT_INTEGRATOR_SPPM
                                         : -1;
         s->icfg.path_length = (int32_t)num(integ->get("path_length"), 6.0f);
         vec3(integ->get("sky_col"), s->icfg.sky_col);
         s->icfg.enable_vm = num(integ->get("enable_vm"), 0.0f) == 1.0f ? 1 : 0;
         s->icfg.radius_factor = num(integ->get("radius_factor"), 0.025f);
+        s->icfg.base_radius = num(integ->get("base_radius"), 0.03f);  // SceneConfig.h:34
     }
     // meshes (LumenScene.cpp:270-334)
     const JVal* mf = root.get("mesh_file");

@@ -23,11 +23,12 @@
 // The light, eye and merge kernels are persistent.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 
 #include "dev_scene.h"
+#include "dev_count.h"
 #include "dev_trace.h"
 #include "vcm.h"
 
@@ -37,10 +38,6 @@ namespace {
 
 constexpr float VCM_TMIN = 0.001f;    // vcm_light.rgen:54
 constexpr float VCM_TMAX = 10000.0f;  // vcm_light.rgen:55
-
-struct Ctr {
-    uint32_t closest, shadow, nodes, tris;
-};
 
 struct VState {  // VCMState (utils.glsl:28-39)
     f3 wi, n_s, pos, thr;
@@ -104,7 +101,7 @@ template <bool COUNT>
 LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk, Ctr& c) {
     if (COUNT) c.shadow++;
     HitRec h;
-    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.nodes, c.tris);
+    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.snodes, c.stris);
 }
 
 template <bool COUNT>
@@ -307,21 +304,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
             }
         }
     }
-    if (COUNT) {
-        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, tt = c.tris;
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o);
-            b += __shfl_xor(b, o);
-            n += __shfl_xor(n, o);
-            tt += __shfl_xor(tt, o);
-        }
-        if (lane == 0) {
-            atomicAdd(&counts[0], a);
-            atomicAdd(&counts[1], b);
-            atomicAdd(&counts[2], n);
-            atomicAdd(&counts[3], tt);
-        }
-    }
+    if (COUNT) flush_counts(c, counts);
 }
 
 // corrected estimator: every photon into its bucket's range
@@ -505,10 +488,15 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
 #ifndef LT_VCM_CELL_DIV
 #define LT_VCM_CELL_DIV 2
 #endif
+// COUNT: the wave tallies the photons it tests (16 B position each), the
+// photons it merges (48 B record each) and the algorithmic bytes of its
+// queries (80-B record + 8 B of bucket range per cell) into counts[4..6].
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_VCM_MERGE_WAVES))) void k_vcm_merge(
     SceneView sv, VcmLaunch L, Grid Gr, const VcmQuery* __restrict__ queries, const uint32_t* __restrict__ n_queries,
-    float4* __restrict__ fcol) {
+    float4* __restrict__ fcol, unsigned long long* __restrict__ counts) {
     const uint32_t lane = threadIdx.x & 63u;
+    unsigned long long n_tested = 0, n_merged = 0, n_bytes = 0;  // wave-uniform
     // photons of the wave's query that passed the distance and cell tests,
     // waiting for a full wave of merges (at most 63 + 64 pending)
     __shared__ uint32_t s_k[kTraceBlock / 64u][128];
@@ -540,6 +528,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         grid_idx_fast(L, qpos + mk3(L.radius), hi);
         const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
         const int ncells = (hi[0] - lo[0] + 1) * ny * nz;
+        if (COUNT) n_bytes += sizeof(VcmQuery) + 8ull * (unsigned long long)ncells;
         f3 part = mk3(0.0f);
         uint32_t pend = 0u;  // wave-uniform
         auto merge_slot = [&](uint32_t slot) {
@@ -569,6 +558,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                 if ((int)lane >= o) incl += v;
             }
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            if (COUNT) n_tested += total;
             for (uint32_t t0 = 0; t0 < total; t0 += 64u) {
                 const uint32_t t = t0 + lane;
                 int j = 0;  // the cell holding photon t: the first lane whose scan exceeds t
@@ -601,6 +591,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                     wd2[slot] = dist_sqr;
                 }
                 pend += (uint32_t)__popcll(m);
+                if (COUNT) n_merged += (unsigned long long)__popcll(m);
                 if (pend >= 64u) {  // one full wave of merges, then the rest moves down
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -642,6 +633,11 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
             atomicAdd(fc + 1, part.y);
             atomicAdd(fc + 2, part.z);
         }
+    }
+    if (COUNT && lane == 0u) {
+        atomicAdd(&counts[4], n_tested);
+        atomicAdd(&counts[5], n_merged);
+        atomicAdd(&counts[6], n_bytes + n_tested * 16ull + n_merged * 48ull);
     }
 }
 
@@ -867,21 +863,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         a.w = 1.0f;
         accum[i] = a;
     }
-    if (COUNT) {
-        unsigned long long a = c.closest, b = c.shadow, n = c.nodes, tt = c.tris;
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o);
-            b += __shfl_xor(b, o);
-            n += __shfl_xor(n, o);
-            tt += __shfl_xor(tt, o);
-        }
-        if (lane == 0) {
-            atomicAdd(&counts[0], a);
-            atomicAdd(&counts[1], b);
-            atomicAdd(&counts[2], n);
-            atomicAdd(&counts[3], tt);
-        }
-    }
+    if (COUNT) flush_counts(c, counts);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -947,8 +929,8 @@ lt_status vcm_alloc(lt_context* ctx, VcmBuffers& b, uint32_t width, uint32_t hei
     LT_HIP(ctx, hipMalloc(&b.ppos, N * nv * sizeof(float4)));
     LT_HIP(ctx, hipMalloc(&b.prest, N * nv * sizeof(VcmPhotonRest)));
     LT_HIP(ctx, hipMalloc(&b.splat, N * 3 * sizeof(float)));
-    LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b.scan_bytes, b.count, b.start, (int)(cells + 1),
-                                                 ctx->stream));
+    LT_HIP(ctx, rocprim::exclusive_scan(nullptr, b.scan_bytes, b.count, b.start, 0u, cells + 1,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
     LT_HIP(ctx, hipMalloc(&b.scan_tmp, b.scan_bytes));
     fill(ctx, (uint32_t*)b.splat, N * 3);
     fill(ctx, b.cnt, N);
@@ -987,10 +969,10 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
         LT_HIP(ctx, hipMalloc(&b.queries, (size_t)b.n_paths * (size_t)b.max_depth * sizeof(VcmQuery)));
         LT_HIP(ctx, hipMalloc(&b.qctr, 2 * sizeof(uint32_t)));
     }
-    const uint32_t merge_blocks = ctx->num_cus * resident_blocks(reinterpret_cast<const void*>(&k_vcm_merge), 0u);
+    const uint32_t merge_blocks = ctx->num_cus * resident_blocks(reinterpret_cast<const void*>(&k_vcm_merge<false>), 0u);
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
-    if (count) fill(ctx, (uint32_t*)d_counts, 8);
+    if (count) fill(ctx, (uint32_t*)d_counts, 2 * kCountSlots);
     const size_t N = b.n_paths, cells = 10 * N;
     const dim3 lgrid(light_blocks), egrid(eye_blocks), block(kTraceBlock);
     const Grid G{b.verts, b.cnt, b.key, b.count, b.start, b.ppos, b.prest};
@@ -1010,6 +992,7 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
         L.eta_vcm = PI * r2 * (float)N;
         L.eta_vc = 1.0f / L.eta_vcm;
         L.eta_vm = L.use_vm ? PI * r2 * (float)N : 0.0f;
+        if (f == 0) phase_mark(ctx);
         fill(ctx, ctx->counters, 2);
         if (L.use_vm) {
             if (!L.corrected) fill(ctx, b.key, cells);
@@ -1022,8 +1005,8 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
             hipLaunchKernelGGL((k_vcm_light<false>), lgrid, block, 0, ctx->stream, sv, L, b.verts, b.cnt, b.splat,
                                b.key, b.count, ctx->counters, ctx->spill, cnt);
         if (L.use_vm && L.corrected) {
-            LT_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, b.scan_bytes, b.count, b.start,
-                                                         (int)(cells + 1), ctx->stream));
+            LT_HIP(ctx, rocprim::exclusive_scan(b.scan_tmp, b.scan_bytes, b.count, b.start, 0u, cells + 1,
+                                                rocprim::plus<uint32_t>(), ctx->stream));
             fill(ctx, b.fill, cells);
             hipLaunchKernelGGL(k_vcm_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, L,
                                b.verts, b.cnt, b.start, b.fill, b.ppos, b.prest);
@@ -1031,15 +1014,24 @@ lt_status launch_vcm(lt_context* ctx, VcmBuffers& b, const VcmLaunch& L0, const 
         float4* fcol = merge_pass ? b.fcol : nullptr;
         VcmQuery* queries = merge_pass ? b.queries : nullptr;
         if (merge_pass) fill(ctx, b.qctr, 1);
+        if (f == 0) phase_mark(ctx);
         if (count)
             hipLaunchKernelGGL((k_vcm_eye<true>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum, fcol,
                                queries, b.qctr, ctx->counters + 1, ctx->spill, cnt);
         else
             hipLaunchKernelGGL((k_vcm_eye<false>), egrid, block, 0, ctx->stream, sv, L, G, b.splat, accum, fcol,
                                queries, b.qctr, ctx->counters + 1, ctx->spill, cnt);
+        if (f == 0) phase_mark(ctx);
         if (merge_pass) {
-            hipLaunchKernelGGL(k_vcm_merge, dim3(merge_blocks), block, 0, ctx->stream, sv, L, G, b.queries, b.qctr,
-                               b.fcol);
+            if (count)
+                hipLaunchKernelGGL(k_vcm_merge<true>, dim3(merge_blocks), block, 0, ctx->stream, sv, L, G, b.queries,
+                                   b.qctr, b.fcol, cnt);
+            else
+                hipLaunchKernelGGL(k_vcm_merge<false>, dim3(merge_blocks), block, 0, ctx->stream, sv, L, G, b.queries,
+                                   b.qctr, b.fcol, cnt);
+        }
+        if (f == 0) phase_mark(ctx);
+        if (merge_pass) {
             hipLaunchKernelGGL(k_vcm_resolve, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, b.fcol,
                                b.splat, accum, (uint32_t)N, L.frame_num);
         }

@@ -223,6 +223,38 @@ class VCM final : public Integrator {
     lt_camera camera_{};
 };
 
+// Stochastic progressive photon mapping (src/RayTracer/SPPM.{h,cpp}): the
+// scene's SPPMConfig base_radius (SceneConfig.h:33-36) and SPPM::update's
+// frame counter (SPPM.cpp:178-185); the per-pixel state restarts at frame 0.
+class SPPM final : public Integrator {
+   public:
+    SPPM(LumenScene* scene, lt_context* ctx, uint32_t w, uint32_t h) : Integrator(scene, ctx, w, h) {}
+    void init() override {
+        check(lt_scene_upload(ctx_, &lumen_scene->desc()), ctx_, "lt_scene_upload");
+        lt_scene_integrator_cfg sc = lumen_scene->config();
+        lt_sppm_config cfg{};
+        cfg.width = w_;
+        cfg.height = h_;
+        cfg.max_depth = sc.path_length > 0 ? sc.path_length : 6;
+        for (int k = 0; k < 3; k++) cfg.sky_col[k] = sc.sky_col[k];
+        cfg.base_radius = sc.base_radius > 0.0f ? sc.base_radius : 0.03f;  // SceneConfig.h:34
+        check(lt_integrator_create(ctx_, LT_INTEGRATOR_SPPM, &cfg, sizeof(cfg), &it_), ctx_, "lt_integrator_create");
+        camera_ = lumen_scene->camera(w_, h_);
+        frame_num = 0;  // SPPM.cpp:60
+    }
+    void render() override {
+        lt_frame_params p{};
+        p.camera = camera_;
+        p.frame_num = frame_num;
+        p.seed = seed;
+        p.spp = 1;
+        check(lt_render(it_, &p), ctx_, "lt_render");
+    }
+
+   private:
+    lt_camera camera_{};
+};
+
 // Options of the factory beyond the scene's own integrator block.
 struct IntegratorOptions {
     uint32_t path_strategy = LT_PATH_MEGAKERNEL;  // Path execution form
@@ -241,6 +273,7 @@ inline std::unique_ptr<Integrator> create_integrator(const std::string& name, Lu
     if (name == "bdpt" || name == "BDPT") return std::unique_ptr<Integrator>(new BDPT(scene, ctx, w, h, o.mis));
     if (name == "vcm" || name == "VCM")
         return std::unique_ptr<Integrator>(new VCM(scene, ctx, w, h, o.vcm_estimator, o.vcm_alpha));
+    if (name == "sppm" || name == "SPPM") return std::unique_ptr<Integrator>(new SPPM(scene, ctx, w, h));
     throw std::runtime_error("integrator '" + name + "' is not built");
 }
 
@@ -249,6 +282,7 @@ inline std::string integrator_name(lt_integrator_kind k) {
     switch (k) {
         case LT_INTEGRATOR_BDPT: return "bdpt";
         case LT_INTEGRATOR_VCM: return "vcm";
+        case LT_INTEGRATOR_SPPM: return "sppm";
         default: return "path";
     }
 }

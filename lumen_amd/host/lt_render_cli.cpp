@@ -7,7 +7,7 @@
 // with --exr-float), otherwise a PFM (little-endian RGB float, rows bottom to top).
 //
 //   lt_render_cli --scene cornell|terrain|caustics|diff_wall|<scene.json>
-//                 [--integrator path|bdpt|vcm] [--width W] [--height H] [--spp N]
+//                 [--integrator path|bdpt|vcm|sppm] [--width W] [--height H] [--spp N]
 //                 [--seed S] [--strategy megakernel|persistent|wavefront]
 //                 [--mis balance|power] [--estimator reference|corrected] [--alpha A]
 //                 [--out image.exr|image.pfm] [--exr-float]

@@ -37,6 +37,36 @@ def test_struct_layouts_match_reference_wire_format():
     assert _abi.RAY_DTYPE.itemsize == 32 and _abi.HIT_DTYPE.itemsize == 20
 
 
+def test_config_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of the config and parameter structs have the sizes
+    and field offsets the C compiler gives the declarations of include/lt.h."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = ["lt_path_config", "lt_bdpt_config", "lt_vcm_config", "lt_sppm_config", "lt_frame_params",
+               "lt_render_stats", "lt_scene_integrator_cfg", "lt_adjoint_config", "lt_camera"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for name in structs:
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for fname, _ in getattr(_abi, name)._fields_:
+            lines.append(f'printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.run([cc, str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for name in structs:
+        st = getattr(_abi, name)
+        assert int(out[name]) == C.sizeof(st), name
+        for fname, _ in st._fields_:
+            assert int(out[f"{name}.{fname}"]) == getattr(st, fname).offset, f"{name}.{fname}"
+
+
 def test_context_without_device_or_with_device():
     h = C.c_void_p()
     st = la.lib().lt_context_create(0, C.byref(h))

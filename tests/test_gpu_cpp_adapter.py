@@ -63,9 +63,10 @@ def test_cli_writes_exr(gpu_ctx):
     ("bdpt", ["--mis", "power"], dict(mis="power")),
     ("vcm", ["--estimator", "corrected"], dict(estimator="corrected")),
     ("vcm", [], dict(estimator="reference")),
+    ("sppm", [], {}),
 ])
 def test_cli_bdpt_vcm_match_python_integrators(gpu_ctx, name, extra, py_kw):
-    """The C++ factory builds BDPT and VCM (RayTracer::create_integrator,
+    """The C++ factory builds BDPT, VCM and SPPM (RayTracer::create_integrator,
     RayTracer.cpp:244-282) from the scene's integrator block; their frames equal
     those of the Python mirror with the same options and seed."""
     print(f"[gpu] cli {name} {extra}", flush=True)

@@ -112,6 +112,17 @@ def test_loads_shapes_materials_lights(tmp_path):
     assert np.allclose(sc.desc.scene_min[:], [-1, 0, -1]) and np.allclose(sc.desc.scene_max[:], [1, 1, 1])
 
 
+def test_sppm_integrator_block(tmp_path):
+    """"type": "sppm" with its base_radius (LumenScene.cpp:248-249); without
+    the key the SPPMConfig default 0.03 (SceneConfig.h:34)."""
+    js = JSON.replace('"type": "vcm"', '"type": "sppm", "base_radius": 0.07')
+    cfg = la.Scene.load_json(str(_write(tmp_path, js=js))).integrator_cfg()
+    assert cfg.kind == 3 and cfg.path_length == 7 and cfg.base_radius == pytest.approx(0.07)
+    js = JSON.replace('"type": "vcm"', '"type": "sppm"')
+    cfg = la.Scene.load_json(str(_write(tmp_path, js=js))).integrator_cfg()
+    assert cfg.kind == 3 and cfg.base_radius == pytest.approx(0.03)
+
+
 @pytest.mark.parametrize("bad,code", [
     ("{\"integrator\": {\"type\": \"path\"}, \"bsdfs\": [], ", "LT_ERR_PARSE"),
     ("{\"bsdfs\": [{\"type\": \"diffuse\", \"texture\": \"a.png\", \"refs\": []}], \"mesh_file\": \"scene.obj\"}",
