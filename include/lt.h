@@ -278,8 +278,14 @@ typedef struct lt_tuning {
     uint32_t bdpt_chunk_tiles;     /* BDPT: 8x8 pixel tiles per walk/connection pass,
                                       read when an integrator is created; default 0 =
                                       as many as fit a 4 GiB vertex store               */
-    uint32_t reserved[10];
+    uint32_t wf_sort;              /* wavefront Path shade stages: LT_WF_SORT_NONE
+                                      (default) or LT_WF_SORT_BSDF (the hit paths of a
+                                      depth grouped by the BSDF type of their material
+                                      before shading); 0 = unchanged                     */
+    uint32_t reserved[9];
 } lt_tuning;
+#define LT_WF_SORT_NONE 1u
+#define LT_WF_SORT_BSDF 2u
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);
 

@@ -32,6 +32,7 @@ LT_INTEGRATOR_PATH, LT_INTEGRATOR_BDPT, LT_INTEGRATOR_VCM, LT_INTEGRATOR_SPPM = 
 LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT, LT_PATH_PERSISTENT = 0, 1, 2
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
+LT_WF_SORT_NONE, LT_WF_SORT_BSDF = 1, 2
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
 
@@ -140,7 +141,7 @@ class lt_procedural_params(C.Structure):
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("regen_min", C.c_uint32),
-                ("bdpt_chunk_tiles", C.c_uint32), ("reserved", C.c_uint32 * 10)]
+                ("bdpt_chunk_tiles", C.c_uint32), ("wf_sort", C.c_uint32), ("reserved", C.c_uint32 * 9)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

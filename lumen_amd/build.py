@@ -7,7 +7,9 @@ and linked into one shared library; the C ABI is include/lt.h.  Run as
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -43,11 +45,30 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build lumen_amd)")
 
 
-def _compile(src: Path) -> Path:
+_REMARK = re.compile(r"remark:\s+(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]"
+                     r"|LDS Size \[bytes/block\]):\s*(\S+)")
+_KEYS = {"VGPRs": "vgprs", "AGPRs": "agprs", "ScratchSize [bytes/lane]": "scratch_bytes",
+         "Occupancy [waves/SIMD]": "waves_per_simd", "LDS Size [bytes/block]": "lds_bytes"}
+
+
+def _resources(remarks: str) -> dict:
+    """Per-kernel register, scratch, LDS and occupancy figures from the
+    compiler's kernel-resource-usage remarks."""
+    out, cur = {}, None
+    for m in _REMARK.finditer(remarks):
+        key, val = m.group(1), m.group(2)
+        if key == "Function Name":
+            cur = out.setdefault(val, {})
+        elif cur is not None:
+            cur[_KEYS[key]] = int(val)
+    return out
+
+
+def _compile(src: Path):
     out = OBJ / (src.name + ".o")
     cmd = [_hipcc(), *COMMON]
     if src.suffix == ".hip":
-        cmd += ["-x", "hip", *DEVICE]
+        cmd += ["-x", "hip", *DEVICE, "-Rpass-analysis=kernel-resource-usage"]
     else:
         # host-only C++ that calls the HIP runtime API
         cmd += ["-D__HIP_PLATFORM_AMD__"]
@@ -55,7 +76,7 @@ def _compile(src: Path) -> Path:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src.name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    return out
+    return out, _resources(r.stderr) if src.suffix == ".hip" else {}
 
 
 def build(verbose: bool = False) -> Path:
@@ -64,7 +85,13 @@ def build(verbose: bool = False) -> Path:
     srcs = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp")])
     jobs = min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        results = list(ex.map(_compile, srcs))
+    objs = [o for o, _ in results]
+    # the device kernels' footprints (tests/test_kernel_footprint.py)
+    res = {}
+    for _, r in results:
+        res.update(r)
+    (OBJ / "kernel_resources.json").write_text(json.dumps(res, indent=1, sort_keys=True))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
            "-lpthread", "-lz"]

@@ -6,17 +6,19 @@
 
 namespace ltd {
 
-// Rays traced and BVH8 work of one lane.  nodes / tris count every query;
-// snodes / stris the part of it that occlusion queries took.
+// Rays traced and BVH8 work of one lane: nodes / tris of closest-hit
+// queries, snodes / stris of occlusion queries.
 struct Ctr {
     uint32_t closest, shadow, nodes, tris, snodes, stris;
 };
 
 // Sums a wave's counters and adds them to counts (kCountSlots u64 slots:
-// closest, shadow, nodes, tris, ..., occlusion nodes at 7, occlusion tris
-// at 8).  Every lane of the wave must call it.
+// closest rays, shadow rays, nodes and tris of all queries, ..., nodes and
+// tris of the occlusion queries at 7 and 8).  Every lane of the wave must
+// call it.
 LT_DEV void flush_counts(const Ctr& c, unsigned long long* counts) {
-    unsigned long long v[6] = {c.closest, c.shadow, c.nodes, c.tris, c.snodes, c.stris};
+    unsigned long long v[6] = {c.closest, c.shadow, (unsigned long long)c.nodes + c.snodes,
+                               (unsigned long long)c.tris + c.stris, c.snodes, c.stris};
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
 #pragma unroll

@@ -291,8 +291,10 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
             }
         }
         if (hit_int) {
-            // nearest internal child is pushed last (popped first)
-            int nearest = -1;
+            // nearest internal child is pushed last (popped first); it starts
+            // at the first hit child, so that it names a hit child even if no
+            // entry distance compares below +inf (an overflowed slab)
+            int nearest = __builtin_ctz(hit_int);
             float tbest = INFINITY;
 #pragma unroll
             for (int c = 0; c < 8; c++)

@@ -6,6 +6,7 @@
 #include "path.h"
 
 constexpr int kWaveMaxDepth = 32;
+constexpr int kWfBins = 8;  // material sort: miss, then one bin per BSDF type, then other
 
 // occlusion query with its pending NEE contribution (48 B)
 struct ShadowRec {
@@ -34,6 +35,8 @@ struct WaveState {
     uint4* meta;  // x = pcg4d counter word, y = last bounce specular
     uint4* hit;   // prim, inst (0xFFFFFFFF = miss), u, v
     uint32_t* queue[2];
+    uint32_t* sorted;  // extend queue of a depth grouped by BSDF type (LT_WF_SORT_BSDF)
+    uint8_t* bin;      // each queue item's BSDF bin
     ShadowRec* shadow;
     MisRec* mis;
 };
@@ -45,6 +48,7 @@ struct WaveCtl {
     uint32_t sh_len[kWaveMaxDepth];
     uint32_t mis_len[kWaveMaxDepth];
     uint32_t work[kWaveMaxDepth][4];
+    uint32_t bins[kWaveMaxDepth][2][kWfBins];  // material sort: items per bin, then scatter fill
 };
 
 struct WaveBuffers {

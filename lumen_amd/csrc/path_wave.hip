@@ -99,9 +99,10 @@ __global__ __launch_bounds__(256) void k_wf_generate(PathLaunch L, uint32_t f, W
 #endif
 template <bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LT_WF_SHADE_WAVES))) void k_wf_shade(SceneView sv, PathLaunch L, uint32_t f, WaveState S,
-                                                  WaveCtl* ctl, uint32_t depth, unsigned long long* __restrict__ counts) {
+                                                  WaveCtl* ctl, uint32_t depth, uint32_t sorted,
+                                                  unsigned long long* __restrict__ counts) {
     const uint32_t frame_word = frame_of(L, f) ^ L.seed;
-    const uint32_t* q = S.queue[depth & 1];
+    const uint32_t* q = sorted ? S.sorted : S.queue[depth & 1];
     uint32_t* qn = S.queue[(depth + 1) & 1];
     uint32_t n_closest = 0, n_shadow = 0;
     WAVE_LOOP(ctl->q_len[depth], &ctl->work[depth][1])
@@ -361,6 +362,68 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
 }
 
 // running mean with the NaN guard (path.rgen:102-112)
+// Material sort (LT_WF_SORT_BSDF): the hit paths of extend queue `depth` are
+// grouped by the BSDF type of their material (bin 0: misses, bins 1-6:
+// diffuse, mirror, glass, dielectric, conductor, principled, bin 7: other),
+// so that the waves of the shade stage run one BSDF's code rather than
+// several types' branches of bsdf_commons.glsl:68-182 one after another.
+// A path's result does not depend on where its shading runs.
+LT_DEV uint32_t bsdf_bin(const SceneView& sv, uint4 hit) {
+    if (hit.y == 0xFFFFFFFFu) return 0u;
+    const uint32_t t = sv.mats[sv.prims[hit.y].material_index].bsdf_type;
+    if (t == 0u || (t & (t - 1u)) != 0u || t > LT_BSDF_PRINCIPLED) return 7u;
+    return 32u - (uint32_t)__builtin_clz(t);  // 1 -> 1, 2 -> 2, ..., 32 -> 6
+}
+
+__global__ __launch_bounds__(256) void k_wf_bin_count(SceneView sv, WaveState S, WaveCtl* ctl, uint32_t depth) {
+    __shared__ uint32_t h[kWfBins];
+    if (threadIdx.x < kWfBins) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t* q = S.queue[depth & 1];
+    const uint32_t n = ctl->q_len[depth];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t b = bsdf_bin(sv, S.hit[q[i]]);
+        S.bin[i] = (uint8_t)b;
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kWfBins && h[threadIdx.x] != 0u) atomicAdd(&ctl->bins[depth][0][threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_wf_bin_scatter(WaveState S, WaveCtl* ctl, uint32_t depth) {
+    const uint32_t* q = S.queue[depth & 1];
+    const uint32_t n = ctl->q_len[depth];
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t start[kWfBins];
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int k = 0; k < kWfBins; k++) {
+        start[k] = acc;
+        acc += ctl->bins[depth][0][k];
+    }
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // every lane of a wave runs the same number of iterations (ballots below)
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        const bool valid = i < n;
+        const uint32_t b = valid ? (uint32_t)S.bin[i] : 0xFFu;
+        uint32_t slot = 0u;
+#pragma unroll
+        for (int k = 0; k < kWfBins; k++) {
+            const unsigned long long m = __ballot(b == (uint32_t)k);
+            if (m == 0ull) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t off = 0u;
+            if ((int)lane == leader) off = atomicAdd(&ctl->bins[depth][1][k], (uint32_t)__popcll(m));
+            off = __shfl(off, leader);
+            if (b == (uint32_t)k) slot = start[k] + off + (uint32_t)__popcll(m & below);
+        }
+        if (valid) S.sorted[slot] = q[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_wf_accumulate(PathLaunch L, uint32_t f, WaveState S,
                                                        float4* __restrict__ accum) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -407,6 +470,8 @@ lt_status wave_alloc(lt_context* ctx, WaveBuffers& wb, uint32_t n_paths) {
     if (e == hipSuccess) e = A((void**)&wb.S.hit, n * 16);
     if (e == hipSuccess) e = A((void**)&wb.S.queue[0], n * 4);
     if (e == hipSuccess) e = A((void**)&wb.S.queue[1], n * 4);
+    if (e == hipSuccess) e = A((void**)&wb.S.sorted, n * 4);
+    if (e == hipSuccess) e = A((void**)&wb.S.bin, n);
     if (e == hipSuccess) e = A((void**)&wb.S.shadow, n * sizeof(ShadowRec));
     if (e == hipSuccess) e = A((void**)&wb.S.mis, n * sizeof(MisRec));
     if (e == hipSuccess) e = A((void**)&wb.ctl, sizeof(WaveCtl));
@@ -420,7 +485,7 @@ lt_status wave_alloc(lt_context* ctx, WaveBuffers& wb, uint32_t n_paths) {
 
 void wave_free(WaveBuffers& wb) {
     void* ptrs[] = {wb.S.ray_o, wb.S.ray_d, wb.S.thr, wb.S.col, wb.S.misc, wb.S.meta, wb.S.hit,
-                    wb.S.queue[0], wb.S.queue[1], wb.S.shadow, wb.S.mis, wb.ctl};
+                    wb.S.queue[0], wb.S.queue[1], wb.S.sorted, wb.S.bin, wb.S.shadow, wb.S.mis, wb.ctl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     wb = WaveBuffers{};
@@ -436,6 +501,7 @@ lt_status launch_path_wavefront(lt_context* ctx, WaveBuffers& wb, const PathLaun
     st = ensure_spill(ctx, tblocks * kTraceBlock);
     if (st != LT_OK) return st;
     const size_t sblocks = (size_t)ctx->num_cus * 8;
+    const uint32_t sort = ctx->tuning.wf_sort == LT_WF_SORT_BSDF ? 1u : 0u;
     SceneView sv = make_scene_view(ctx);
     auto* cnt = (unsigned long long*)d_counts;
     if (count) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, (uint32_t)kCountSlots);
@@ -455,12 +521,16 @@ lt_status launch_path_wavefront(lt_context* ctx, WaveBuffers& wb, const PathLaun
             else
                 hipLaunchKernelGGL((k_wf_trace<false>), dim3(tblocks), dim3(kTraceBlock), 0, ctx->stream, sv, wb.S,
                                    wb.ctl, d, d, 1u, d - with_sec, with_sec, ctx->spill, cnt);
+            if (sort) {
+                hipLaunchKernelGGL(k_wf_bin_count, dim3(sblocks), dim3(256), 0, ctx->stream, sv, wb.S, wb.ctl, d);
+                hipLaunchKernelGGL(k_wf_bin_scatter, dim3(sblocks), dim3(256), 0, ctx->stream, wb.S, wb.ctl, d);
+            }
             if (count)
                 hipLaunchKernelGGL((k_wf_shade<true>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, f, wb.S,
-                                   wb.ctl, d, cnt);
+                                   wb.ctl, d, sort, cnt);
             else
                 hipLaunchKernelGGL((k_wf_shade<false>), dim3(sblocks), dim3(256), 0, ctx->stream, sv, L, f, wb.S,
-                                   wb.ctl, d, cnt);
+                                   wb.ctl, d, sort, cnt);
         }
         hipLaunchKernelGGL(k_wf_accumulate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, L, f, wb.S, accum);
     }

@@ -122,7 +122,7 @@ extern "C" void ora_bvh8_trace(const void* nodes_v, uint32_t n_nodes, const void
                 }
             }
             if (done || !hit_int) continue;
-            int nearest = -1;
+            int nearest = __builtin_ctz(hit_int);
             float tb = INFINITY;
             for (int c = 0; c < 8; c++)
                 if (((hit_int >> c) & 1u) && tn[c] < tb) {

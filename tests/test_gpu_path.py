@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import lumen_amd as la
+from lumen_amd import _abi
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -92,6 +93,31 @@ def test_execution_forms_equal_megakernel(gpu_ctx, kind, W, H, frames, kw, strat
     assert np.all(d <= 1e-4 * (1.0 + np.abs(a[..., :3]))), d.max()
 
 
+@pytest.mark.parametrize("kind,kw", [
+    ("cornell", {}),
+    ("caustics", {}),
+    ("terrain", dict(terrain_tris=20000, num_instances=100, num_point_lights=8)),
+])
+def test_wavefront_material_sort_is_exact(gpu_ctx, kind, kw):
+    """LT_WF_SORT_BSDF only moves where a path is shaded (the hit paths of a
+    depth grouped by BSDF type): every path draws the same random numbers and
+    writes only its own state, so the image is bit-identical."""
+    print(f"[gpu] wavefront material sort {kind}", flush=True)
+    sc = la.Scene.procedural(kind, seed=1, **kw)
+    imgs = []
+    for mode in (_abi.LT_WF_SORT_NONE, _abi.LT_WF_SORT_BSDF):
+        gpu_ctx.set_tuning(wf_sort=mode)
+        it = la.Path(sc, gpu_ctx, 72, 40, strategy="wavefront")
+        it.init()
+        it.create_accel()
+        it.render(spp=2)
+        imgs.append(it.output_tex)
+        it.destroy()
+    gpu_ctx.set_tuning(wf_sort=_abi.LT_WF_SORT_NONE)
+    assert np.any(imgs[0][..., :3] > 0)
+    assert np.array_equal(imgs[0], imgs[1])
+
+
 @pytest.mark.parametrize("strategy", ["megakernel", "persistent", "wavefront"])
 def test_corrected_nee_pick_replays_oracle(gpu_ctx, strategy):
     """LT_NEE_CORRECTED: the NEE sample weighted by its own pick probability,
@@ -112,3 +138,27 @@ def test_corrected_nee_pick_replays_oracle(gpu_ctx, strategy):
     rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
     print(f"[gpu] corrected nee {strategy}: pixels matching {ok.mean():.4f}, rel rmse {rel:.2e}", flush=True)
     assert ok.mean() >= 0.995 and rel <= 0.01
+
+
+def test_counting_totals_are_consistent(gpu_ctx):
+    """A counting frame of the megakernel and of the persistent kernel trace
+    the same paths with the same traversal: equal ray, node and triangle
+    counts; the occlusion queries' part of the nodes and triangles is
+    positive and below the totals (the terrain has point lights)."""
+    print("[gpu] counting totals", flush=True)
+    sc = la.Scene.procedural("terrain", seed=1, terrain_tris=20000, num_instances=100, num_point_lights=8)
+    stats = []
+    for strat in ("megakernel", "persistent"):
+        it = la.Path(sc, gpu_ctx, 96, 54, strategy=strat)
+        it.init()
+        it.create_accel()
+        it.set_counting(True)
+        it.render(spp=1)
+        s = it.stats()
+        stats.append((s.closest_rays, s.shadow_rays, s.nodes_visited, s.tris_tested, s.shadow_nodes_visited,
+                      s.shadow_tris_tested))
+        it.destroy()
+    assert stats[0] == stats[1]
+    closest, shadow, nodes, tris, snodes, stris = stats[0]
+    assert closest >= 96 * 54 and shadow > 0
+    assert 0 < snodes < nodes and 0 < stris < tris
