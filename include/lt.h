@@ -155,6 +155,17 @@ typedef struct lt_scene_desc {
     int32_t dir_light_idx;             /* -1 if none, LumenScene.cpp:107-109 */
     float scene_min[3];                /* m_dimensions, LumenScene.cpp:732-751 */
     float scene_max[3];
+    /* Per prim mesh, optional (NULL = identity for every mesh, as JSON scenes
+     * load, LumenScene.cpp:328): LumenPrimMesh::world_matrix (LumenScene.h:32),
+     * 16 floats column-major (glm), the object-to-world transform of the mesh's
+     * TLAS instance (Integrator.cpp:147-158).  Hits report world-space
+     * positions and normals as ray.rchit:62-80 computes them; area lights on
+     * the mesh sample world-space points (commons.glsl:112-149). */
+    const float* world_matrices;
+    /* Per prim mesh, optional: its index count.  Meshes that are instances of
+     * one geometry share index_offset / vertex_offset and differ in their world
+     * matrix; NULL = each mesh's indices end where the next mesh's begin. */
+    const uint32_t* index_counts;
 } lt_scene_desc;
 
 /* SceneUBO subset used by the kernels (commons.h:180-192): column-major

@@ -66,21 +66,38 @@ class Scene:
         check(lib().lt_scene_integrator(self._h, C.byref(cfg)), what="lt_scene_integrator")
         return cfg
 
+    def _mesh_ranges(self):
+        """(index_offset, index_count, vertex_offset) per prim mesh."""
+        d = self.desc
+        pinfo = np.ctypeslib.as_array(C.cast(d.prim_infos, C.POINTER(C.c_uint32)), shape=(d.num_prim_infos, 12))
+        if d.index_counts:
+            counts = np.ctypeslib.as_array(d.index_counts, shape=(d.num_prim_infos,)).astype(np.int64)
+        else:
+            offs = list(pinfo[:, 0].astype(np.int64)) + [d.num_indices]
+            counts = np.diff(offs)
+        return pinfo[:, 0].astype(np.int64), counts, pinfo[:, 1].astype(np.int64)
+
     @property
     def num_triangles(self) -> int:
-        return self.desc.num_indices // 3
+        """Triangles over every mesh, each instance of a shared geometry counted."""
+        return int(self._mesh_ranges()[1].sum() // 3)
 
     def triangles(self) -> np.ndarray:
-        """World-space vertex positions per triangle, shape (T, 3, 3)."""
+        """World-space vertex positions per triangle (mesh by mesh, instances
+        through their world matrices), shape (T, 3, 3)."""
         d = self.desc
         vtx = np.ctypeslib.as_array(C.cast(d.vertices, C.POINTER(C.c_float)), shape=(d.num_vertices, 8))
         idx = np.ctypeslib.as_array(d.indices, shape=(d.num_indices,)).astype(np.int64)
-        pinfo = np.ctypeslib.as_array(C.cast(d.prim_infos, C.POINTER(C.c_uint32)), shape=(d.num_prim_infos, 12))
-        voff = np.zeros(d.num_indices, np.int64)
-        offs = list(pinfo[:, 0]) + [d.num_indices]
-        for m in range(d.num_prim_infos):
-            voff[offs[m]:offs[m + 1]] = pinfo[m, 1]
-        return vtx[idx + voff, :3].reshape(-1, 3, 3)
+        world = (np.ctypeslib.as_array(d.world_matrices, shape=(d.num_prim_infos, 16)) if d.world_matrices
+                 else None)
+        out = []
+        for m, (off, cnt, vo) in enumerate(zip(*self._mesh_ranges())):
+            p = vtx[idx[off:off + cnt] + vo, :3]
+            if world is not None:
+                W = world[m].reshape(4, 4).T.astype(np.float64)  # column-major
+                p = (p.astype(np.float64) @ W[:3, :3].T + W[:3, 3]).astype(np.float32)
+            out.append(p.reshape(-1, 3, 3))
+        return np.concatenate(out) if out else np.zeros((0, 3, 3), np.float32)
 
     def close(self):
         if self._h:

@@ -74,7 +74,8 @@ class lt_scene_desc(C.Structure):
                 ("textures", C.POINTER(lt_texture)), ("num_textures", C.c_uint32),
                 ("roughness_texture_ids", C.POINTER(C.c_int32)), ("spot_cones", C.POINTER(C.c_float)),
                 ("total_light_triangle_cnt", C.c_uint32), ("total_light_area", C.c_float),
-                ("dir_light_idx", C.c_int32), ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3)]
+                ("dir_light_idx", C.c_int32), ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3),
+                ("world_matrices", C.POINTER(C.c_float)), ("index_counts", C.POINTER(C.c_uint32))]
 
 
 class lt_camera(C.Structure):

@@ -37,37 +37,59 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-// one thread per triangle: find the mesh by binary search over index offsets
+// object-to-world point of the transform rows xf (3 x 4), evaluated without
+// contraction in the order the host replay uses
+__device__ __forceinline__ void xf_point(const float* __restrict__ xf, const float* p, float* o) {
+#pragma clang fp contract(off)
+    for (int i = 0; i < 3; i++) o[i] = ((xf[4 * i] * p[0] + xf[4 * i + 1] * p[1]) + xf[4 * i + 2] * p[2]) + xf[4 * i + 3];
+}
+
+// One thread per triangle g of the flattened triangle list (mesh by mesh,
+// instances of one geometry each with their own triangles): the mesh by
+// binary search over the meshes' first triangles; vertices in world space
+// through the mesh's transform, the flattening that gives one BVH the
+// instance semantics of the reference's TLAS (Integrator.cpp:147-158).
 __global__ void k_make_tris(const lt_vertex* __restrict__ vtx, const uint32_t* __restrict__ idx,
-                            const lt_prim_mesh_info* __restrict__ prims, uint32_t nprims, uint32_t ntri,
-                            TriRec* __restrict__ out, uint32_t* __restrict__ cbounds /* 6 ordered uints */) {
+                            const lt_prim_mesh_info* __restrict__ prims, const uint32_t* __restrict__ tri_start,
+                            const float* __restrict__ xf, uint32_t nprims, uint32_t ntri, TriRec* __restrict__ out,
+                            uint32_t* __restrict__ cbounds /* 6 ordered uints */) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (g < ntri) {
-        uint32_t lo = 0, hi = nprims;  // last mesh with index_offset <= 3g
+        uint32_t lo = 0, hi = nprims;  // last mesh whose first triangle is <= g
         while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
-            if (prims[mid].index_offset <= 3 * g)
+            if (tri_start[mid] <= g)
                 lo = mid;
             else
                 hi = mid;
         }
         const lt_prim_mesh_info pi = prims[lo];
-        const uint32_t io = 3 * g;
+        const uint32_t prim = g - tri_start[lo];
+        const uint32_t io = pi.index_offset + 3 * prim;
         TriRec r;
         const float* a = vtx[idx[io] + pi.vertex_offset].pos;
         const float* b = vtx[idx[io + 1] + pi.vertex_offset].pos;
         const float* c = vtx[idx[io + 2] + pi.vertex_offset].pos;
+        if (xf) {
+            const float* m = xf + (size_t)kXfFloats * lo;
+            xf_point(m, a, r.v0);
+            xf_point(m, b, r.v1);
+            xf_point(m, c, r.v2);
+        } else {
+            for (int k = 0; k < 3; k++) {
+                r.v0[k] = a[k];
+                r.v1[k] = b[k];
+                r.v2[k] = c[k];
+            }
+        }
         for (int k = 0; k < 3; k++) {
-            r.v0[k] = a[k];
-            r.v1[k] = b[k];
-            r.v2[k] = c[k];
-            float cen = (a[k] + b[k] + c[k]) * (1.0f / 3.0f);
+            float cen = (r.v0[k] + r.v1[k] + r.v2[k]) * (1.0f / 3.0f);
             clo[k] = cen;
             chi[k] = cen;
         }
         r.gid = g;
-        r.prim = (io - pi.index_offset) / 3;
+        r.prim = prim;
         r.inst = lo;
         out[g] = r;
     }
@@ -389,7 +411,7 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
     if (bvh.nodes) (void)hipFree(bvh.nodes);
     if (bvh.tris) (void)hipFree(bvh.tris);
     bvh = DeviceBVH{};
-    const uint32_t n = ctx->scene.ni / 3;
+    const uint32_t n = ctx->scene.ntris;
     LT_HIP(ctx, hipEventRecord(ctx->ev0, st));
     if (n == 0) {
         bvh.built = true;
@@ -446,7 +468,7 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
         const int B = 256;
         const uint32_t G = (n + B - 1) / B;
         hipLaunchKernelGGL(k_make_tris, dim3(G), dim3(B), 0, st, ctx->scene.vtx, ctx->scene.idx, ctx->scene.prims,
-                           ctx->scene.nprims, n, tris_unsorted, cb);
+                           ctx->scene.tri_start, ctx->scene.xf, ctx->scene.nprims, n, tris_unsorted, cb);
         BCHK(hipGetLastError());
         hipLaunchKernelGGL(k_morton, dim3(G), dim3(B), 0, st, tris_unsorted, n, cb, keys, vals);
         BCHK(hipGetLastError());

@@ -2,7 +2,10 @@
 // integrators and readback.  No exception crosses the ABI.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
+#include <set>
+#include <tuple>
 #include <new>
 #include <string>
 #include <vector>
@@ -33,6 +36,8 @@ SceneView make_scene_view(const lt_context* ctx) {
     v.texels = s.texels;
     v.texs = s.texs;
     v.rough = s.rough;
+    v.xf = s.xf;
+    v.lxf = s.lxf;
     v.nodes = ctx->bvh.nodes;
     v.tris = ctx->bvh.tris;
     v.num_lights = s.nlights;
@@ -67,7 +72,7 @@ uint32_t resident_blocks(const void* kernel, uint32_t fixed) {
 }
 
 static void free_scene(DeviceScene& s) {
-    void* ptrs[] = {s.vtx, s.idx, s.prims, s.mats, s.lights, s.cones, s.texels, s.texs, s.rough};
+    void* ptrs[] = {s.vtx, s.idx, s.prims, s.mats, s.lights, s.cones, s.texels, s.texs, s.rough, s.xf, s.lxf, s.tri_start};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s = DeviceScene{};
@@ -76,6 +81,53 @@ static void free_bvh(DeviceBVH& b) {
     if (b.nodes) (void)hipFree(b.nodes);
     if (b.tris) (void)hipFree(b.tris);
     b = DeviceBVH{};
+}
+
+// Index range of mesh m: its index_offset and either index_counts[m] or up
+// to the next mesh's offset.  false if the range is malformed.
+static bool mesh_range(const lt_scene_desc* s, uint32_t m, uint64_t& off, uint64_t& cnt) {
+    off = s->prim_infos[m].index_offset;
+    if (s->index_counts) {
+        cnt = s->index_counts[m];
+    } else {
+        const uint64_t end = (m + 1 < s->num_prim_infos) ? s->prim_infos[m + 1].index_offset : s->num_indices;
+        if (end < off) return false;
+        cnt = end - off;
+    }
+    return cnt % 3 == 0 && off + cnt <= s->num_indices;
+}
+
+// The transform record of a column-major 4x4 matrix (kXfFloats floats): the
+// object-to-world rows (3 x 4) and the normal matrix, the inverse transpose
+// of the linear part, from its cofactors in double precision.  false if the
+// matrix is not finite or its linear part is singular.
+static bool make_xf(const float* m, float* out, bool& identity) {
+    identity = true;
+    for (int k = 0; k < 16; k++) {
+        if (!std::isfinite(m[k])) return false;
+        if (m[k] != ((k % 5 == 0) ? 1.0f : 0.0f)) identity = false;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) out[4 * i + j] = m[j * 4 + i];
+    double a[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = m[j * 4 + i];
+    double c[3][3];
+    c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    c[0][1] = -(a[1][0] * a[2][2] - a[1][2] * a[2][0]);
+    c[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    c[1][0] = -(a[0][1] * a[2][2] - a[0][2] * a[2][1]);
+    c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+    c[1][2] = -(a[0][0] * a[2][1] - a[0][1] * a[2][0]);
+    c[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+    c[2][1] = -(a[0][0] * a[1][2] - a[0][2] * a[1][0]);
+    c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+    const double det = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
+    if (!(std::fabs(det) > 1e-30)) return false;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[12 + 3 * i + j] = (float)(c[i][j] / det);
+    out[21] = out[22] = out[23] = 0.0f;
+    return true;
 }
 
 template <class T>
@@ -154,27 +206,51 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
         (s->num_lights && !s->lights) || (s->num_textures && !s->textures))
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene array pointer is NULL");
     // validate indices and references on the host: kernels assume them in range
+    std::vector<uint32_t> tri_start(s->num_prim_infos + 1, 0u);
+    std::set<std::tuple<uint64_t, uint64_t, uint32_t>> checked;  // geometry shared by instances: once
     for (uint32_t m = 0; m < s->num_prim_infos; m++) {
         const lt_prim_mesh_info& pi = s->prim_infos[m];
-        uint32_t end = (m + 1 < s->num_prim_infos) ? s->prim_infos[m + 1].index_offset : s->num_indices;
-        if (pi.index_offset > end || (end - pi.index_offset) % 3 != 0 || pi.material_index >= s->num_materials)
+        uint64_t off, cnt;
+        if (!mesh_range(s, m, off, cnt) || pi.material_index >= s->num_materials)
             return lt_fail(ctx, LT_ERR_INVALID_ARG, "prim_infos[" + std::to_string(m) + "] out of range");
-        for (uint32_t i = pi.index_offset; i < end; i++)
+        if ((uint64_t)tri_start[m] + cnt / 3 >= (1ull << 31))
+            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "more than 2^31 triangles over all meshes");
+        tri_start[m + 1] = tri_start[m] + (uint32_t)(cnt / 3);
+        if (!checked.insert({off, cnt, pi.vertex_offset}).second) continue;
+        for (uint64_t i = off; i < off + cnt; i++)
             if ((uint64_t)s->indices[i] + pi.vertex_offset >= s->num_vertices)
                 return lt_fail(ctx, LT_ERR_INVALID_ARG, "index " + std::to_string(i) + " out of range");
     }
-    if (s->num_prim_infos && s->prim_infos[0].index_offset != 0)
+    if (!s->index_counts && s->num_prim_infos && s->prim_infos[0].index_offset != 0)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "prim_infos[0].index_offset must be 0");
+    // transforms: uploaded only where some mesh or light is not the identity
+    std::vector<float> xf, lxf;
+    bool any_xf = false, any_lxf = false;
+    if (s->world_matrices) {
+        xf.resize((size_t)kXfFloats * s->num_prim_infos);
+        for (uint32_t m = 0; m < s->num_prim_infos; m++) {
+            bool id;
+            if (!make_xf(s->world_matrices + 16 * (size_t)m, xf.data() + (size_t)kXfFloats * m, id))
+                return lt_fail(ctx, LT_ERR_INVALID_ARG, "world_matrices[" + std::to_string(m) + "] is singular or not finite");
+            any_xf |= !id;
+        }
+    }
+    lxf.resize((size_t)kXfFloats * s->num_lights);
+    for (uint32_t l = 0; l < s->num_lights; l++) {
+        bool id;
+        if (!make_xf(s->lights[l].world_matrix, lxf.data() + (size_t)kXfFloats * l, id))
+            return lt_fail(ctx, LT_ERR_INVALID_ARG, "lights[" + std::to_string(l) + "].world_matrix is singular or not finite");
+        any_lxf |= !id && (s->lights[l].light_flags & 7u) == LT_LIGHT_AREA;
+    }
     if (s->num_indices && s->num_prim_infos == 0) return lt_fail(ctx, LT_ERR_INVALID_ARG, "indices without meshes");
     for (uint32_t l = 0; l < s->num_lights; l++) {
         const lt_light& L = s->lights[l];
         if ((L.light_flags & 7u) == LT_LIGHT_AREA) {
             if (L.prim_mesh_idx >= s->num_prim_infos)
                 return lt_fail(ctx, LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": mesh out of range");
-            const lt_prim_mesh_info& pi = s->prim_infos[L.prim_mesh_idx];
-            uint32_t end = (L.prim_mesh_idx + 1 < s->num_prim_infos) ? s->prim_infos[L.prim_mesh_idx + 1].index_offset
-                                                                     : s->num_indices;
-            if (L.num_triangles == 0 || pi.index_offset + 3ull * L.num_triangles > end)
+            uint64_t off, cnt;
+            mesh_range(s, L.prim_mesh_idx, off, cnt);
+            if (L.num_triangles == 0 || 3ull * L.num_triangles > cnt)
                 return lt_fail(ctx, LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": triangle count");
         }
     }
@@ -217,6 +293,10 @@ lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
     if ((st = upload(ctx, &d.texels, pool.data(), pool.size())) != LT_OK) return st;
     if (s->roughness_texture_ids)
         if ((st = upload(ctx, &d.rough, s->roughness_texture_ids, s->num_materials)) != LT_OK) return st;
+    if (any_xf && (st = upload(ctx, &d.xf, xf.data(), xf.size())) != LT_OK) return st;
+    if (any_lxf && (st = upload(ctx, &d.lxf, lxf.data(), lxf.size())) != LT_OK) return st;
+    if ((st = upload(ctx, &d.tri_start, tri_start.data(), tri_start.size())) != LT_OK) return st;
+    d.ntris = tri_start.back();
     d.nv = s->num_vertices;
     d.ni = s->num_indices;
     d.nprims = s->num_prim_infos;

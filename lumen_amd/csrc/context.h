@@ -25,6 +25,13 @@ struct SceneView {
     const float4* texels;
     const TexDesc* texs;
     const int32_t* rough;    // per material
+    // per mesh, kXfFloats each (NULL: every mesh has the identity transform):
+    // object-to-world rows (3 x 4), then the normal matrix rows (3 x 3,
+    // inverse transpose of the linear part)
+    const float* xf;
+    // per light, kXfFloats each (NULL: every light's world_matrix is the
+    // identity): the same two matrices from lt_light::world_matrix
+    const float* lxf;
     const ltb::Node8* nodes;
     const ltb::TriRec* tris;
     uint32_t num_lights;
@@ -36,6 +43,10 @@ struct SceneView {
     float scene_min[3], scene_max[3];
 };
 
+// floats per mesh / light transform record: object-to-world 3 x 4, normal
+// matrix 3 x 3, 3 unused
+constexpr int kXfFloats = 24;
+
 struct DeviceScene {
     lt_vertex* vtx = nullptr;
     uint32_t* idx = nullptr;
@@ -46,7 +57,11 @@ struct DeviceScene {
     float4* texels = nullptr;
     TexDesc* texs = nullptr;
     int32_t* rough = nullptr;
+    float* xf = nullptr;           // per mesh transforms (SceneView::xf), or NULL
+    float* lxf = nullptr;          // per light transforms (SceneView::lxf), or NULL
+    uint32_t* tri_start = nullptr; // nprims + 1: first flattened triangle of each mesh
     uint32_t nv = 0, ni = 0, nprims = 0, nmats = 0, nlights = 0, ntex = 0;
+    uint32_t ntris = 0;            // triangles over every mesh (instances counted apart)
     uint32_t total_light_triangle_cnt = 0;
     int32_t dir_light_idx = -1;
     float total_light_area = 0.0f;

@@ -18,7 +18,25 @@ struct Payload {
     bool valid;
 };
 
-// ray.rchit:24-84 with identity object<->world transforms
+// A mesh's transform record (kXfFloats floats, SceneView::xf): rows of the
+// object-to-world 3 x 4 matrix, then rows of the normal matrix (the inverse
+// transpose of its linear part).
+LT_DEV f3 xf_point(const float* m, f3 p) {
+    return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+            m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+LT_DEV f3 xf_vector(const float* m, f3 d) {
+    return {m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+            m[8] * d.x + m[9] * d.y + m[10] * d.z};
+}
+LT_DEV f3 xf_normal(const float* m, f3 n) {
+    return {m[12] * n.x + m[13] * n.y + m[14] * n.z, m[15] * n.x + m[16] * n.y + m[17] * n.z,
+            m[18] * n.x + m[19] * n.y + m[20] * n.z};
+}
+
+// ray.rchit:24-84: barycentric interpolation in object space, then the
+// instance's object-to-world transform for the position and its inverse
+// transpose for the normals (identity when the scene has no transforms)
 LT_DEV void fill_payload(const SceneView& sv, uint32_t inst, uint32_t prim, float u, float v, Payload& p) {
     const lt_prim_mesh_info pi = sv.prims[inst];
     const uint32_t io = pi.index_offset + 3 * prim;
@@ -29,11 +47,20 @@ LT_DEV void fill_payload(const SceneView& sv, uint32_t inst, uint32_t prim, floa
     const float b0 = 1.0f - u - v;
     p.pos = v0 * b0 + v1 * u + v2 * v;
     const f3 nrm = normalize(ld3(a.normal) * b0 + ld3(b.normal) * u + ld3(c.normal) * v);
-    p.n_s = normalize(nrm);
     p.uv = {a.uv0[0] * b0 + b.uv0[0] * u + c.uv0[0] * v, a.uv0[1] * b0 + b.uv0[1] * u + c.uv0[1] * v};
-    const f3 cg = cross(v2 - v0, v1 - v0);
-    p.n_g = normalize(cg);
-    p.area = 0.5f * length(cg);
+    const f3 e0 = v2 - v0, e1 = v1 - v0;
+    const f3 cg = cross(e0, e1);
+    if (sv.xf) {
+        const float* m = sv.xf + (size_t)kXfFloats * inst;
+        p.pos = xf_point(m, p.pos);
+        p.n_s = normalize(xf_normal(m, nrm));
+        p.n_g = normalize(xf_normal(m, cg));
+        p.area = 0.5f * length(cross(xf_vector(m, e0), xf_vector(m, e1)));
+    } else {
+        p.n_s = normalize(nrm);
+        p.n_g = normalize(cg);
+        p.area = 0.5f * length(cg);
+    }
     p.material_idx = pi.material_index;
     p.triangle_idx = prim;
     p.instance_idx = inst;
@@ -92,9 +119,13 @@ struct TriangleRecord {
     float triangle_pdf;
 };
 
-// commons.glsl:112-149 (identity world matrices)
+// commons.glsl:112-149: world = the light's world matrix (column-major),
+// lxf = its transform record or NULL for the identity.  The reference
+// transforms the edges as points (w = 1, :136-137), which only matters for a
+// translated light; here they are vectors (w = 0), so the area pdf is the
+// world triangle's.
 LT_DEV TriangleRecord sample_triangle(const SceneView& sv, const lt_prim_mesh_info& pi, float rx, float ry,
-                                      uint32_t tri, const float* world, f2& uv) {
+                                      uint32_t tri, const float* world, const float* lxf, f2& uv) {
     TriangleRecord r;
     const uint32_t io = pi.index_offset + 3 * tri;
     const lt_vertex a = sv.vtx[sv.idx[io] + pi.vertex_offset];
@@ -105,12 +136,12 @@ LT_DEV TriangleRecord sample_triangle(const SceneView& sv, const lt_prim_mesh_in
     uv = {1.0f - sr, ry * sr};
     const f3 bc = {1.0f - uv.x - uv.y, uv.x, uv.y};
     const f3 d1 = v1 - v0, d2 = v2 - v0;
-    const f4 e0 = mul4(world, f4{d1.x, d1.y, d1.z, 1.0f});
-    const f4 e1 = mul4(world, f4{d2.x, d2.y, d2.z, 1.0f});
+    const f4 e0 = mul4(world, f4{d1.x, d1.y, d1.z, 0.0f});
+    const f4 e1 = mul4(world, f4{d2.x, d2.y, d2.z, 0.0f});
     const f3 pos = v0 * bc.x + v1 * bc.y + v2 * bc.z;
     const f3 nrm = normalize(ld3(a.normal) * bc.x + ld3(b.normal) * bc.y + ld3(c.normal) * bc.z);
     const f4 wp = mul4(world, f4{pos.x, pos.y, pos.z, 1.0f});
-    r.n_s = normalize(nrm);
+    r.n_s = lxf ? normalize(xf_normal(lxf, nrm)) : normalize(nrm);
     r.triangle_pdf = 2.0f / length(cross(mk3(e0.x, e0.y, e0.z), mk3(e1.x, e1.y, e1.z)));
     r.pos = {wp.x, wp.y, wp.z};
     return r;
@@ -146,7 +177,8 @@ LT_DEV f3 sample_light_Li(const SceneView& sv, f4 rp, f3 p, float& pdf_pos_w, f3
     if (lt == LT_LIGHT_AREA) {
         const lt_prim_mesh_info pi = sv.prims[L.prim_mesh_idx];
         rec.triangle_idx = (uint32_t)(rp.y * (float)L.num_triangles);
-        TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, rec.triangle_idx, L.world_matrix, rec.bary);
+        TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, rec.triangle_idx, L.world_matrix,
+                                           sv.lxf ? sv.lxf + (size_t)kXfFloats * rec.light_idx : nullptr, rec.bary);
         Mat lm = load_material(sv, pi.material_index, rec.bary);
         wi = r.pos - p;
         const float d2 = dot(wi, wi);
@@ -278,7 +310,8 @@ LT_DEV f3 sample_light_Le(const SceneView& sv, f4 rp, float rd0, float rd1, floa
         const lt_prim_mesh_info pi = sv.prims[L.prim_mesh_idx];
         const uint32_t tri = (uint32_t)(rp.y * (float)L.num_triangles);
         f2 bary;
-        const TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, tri, L.world_matrix, bary);
+        const TriangleRecord r = sample_triangle(sv, pi, rp.z, rp.w, tri, L.world_matrix,
+                                                 sv.lxf ? sv.lxf + (size_t)kXfFloats * li : nullptr, bary);
         const Mat lm = load_material(sv, pi.material_index, bary);
         pos = r.pos;
         f3 T, B;

@@ -483,6 +483,8 @@ lt_status lt_scene_view(const lt_host_scene* s, lt_scene_desc* d) {
     d->dir_light_idx = s->dir_light_idx;
     memcpy(d->scene_min, s->smin, 12);
     memcpy(d->scene_max, s->smax, 12);
+    d->world_matrices = s->world.empty() ? nullptr : s->world.data();
+    d->index_counts = s->index_counts.empty() ? nullptr : s->index_counts.data();
     return LT_OK;
 }
 

@@ -382,10 +382,20 @@ __global__ __launch_bounds__(256) void k_wf_bin_count(SceneView sv, WaveState S,
     const uint32_t* q = S.queue[depth & 1];
     const uint32_t n = ctl->q_len[depth];
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t b = bsdf_bin(sv, S.hit[q[i]]);
-        S.bin[i] = (uint8_t)b;
-        atomicAdd(&h[b], 1u);
+    const uint32_t lane = threadIdx.x & 63u;
+    // wave-uniform trip count: the ballots below see the whole wave
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+        const uint32_t i = i0 + lane;
+        uint32_t b = 0xFFu;
+        if (i < n) {
+            b = bsdf_bin(sv, S.hit[q[i]]);
+            S.bin[i] = (uint8_t)b;
+        }
+#pragma unroll
+        for (int k = 0; k < kWfBins; k++) {
+            const uint32_t c = (uint32_t)__popcll(__ballot(b == (uint32_t)k));
+            if (lane == 0u && c != 0u) atomicAdd(&h[k], c);
+        }
     }
     __syncthreads();
     if (threadIdx.x < kWfBins && h[threadIdx.x] != 0u) atomicAdd(&ctl->bins[depth][0][threadIdx.x], h[threadIdx.x]);

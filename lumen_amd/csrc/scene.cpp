@@ -264,11 +264,52 @@ uint32_t add_mesh(lt_host_scene& s, const std::string& name, const std::vector<f
     }
     s.prims.push_back(pi);
     s.prim_names.push_back(name);
+    if (!s.index_counts.empty()) {  // the scene has instances: this mesh's own range, identity transform
+        s.index_counts.push_back((uint32_t)nv);
+        for (int k = 0; k < 16; k++) s.world.push_back(k % 5 == 0 ? 1.0f : 0.0f);
+    }
+    return (uint32_t)s.prims.size() - 1;
+}
+
+uint32_t mesh_index_count(const lt_host_scene& s, uint32_t m) {
+    if (!s.index_counts.empty()) return s.index_counts[m];
+    const uint32_t end = (m + 1 < s.prims.size()) ? s.prims[m + 1].index_offset : (uint32_t)s.indices.size();
+    return end - s.prims[m].index_offset;
+}
+
+// the per-mesh transform and index-count arrays, filled for the meshes so
+// far (their own index ranges, identity transforms)
+static void ensure_instancing(lt_host_scene& s) {
+    if (!s.index_counts.empty()) return;
+    std::vector<uint32_t> counts;
+    for (uint32_t m = 0; m < s.prims.size(); m++) counts.push_back(mesh_index_count(s, m));
+    s.index_counts = counts;
+    s.world.assign(16 * s.prims.size(), 0.0f);
+    for (size_t m = 0; m < s.prims.size(); m++)
+        for (int i = 0; i < 4; i++) s.world[16 * m + 5 * i] = 1.0f;
+}
+
+void set_world(lt_host_scene& s, uint32_t m, const float world[16]) {
+    ensure_instancing(s);
+    memcpy(&s.world[16 * (size_t)m], world, 64);
+}
+
+uint32_t add_instance(lt_host_scene& s, const std::string& name, uint32_t proto, uint32_t material,
+                      const float world[16]) {
+    ensure_instancing(s);
+    lt_prim_mesh_info pi = s.prims[proto];
+    pi.material_index = material;
+    s.index_counts.push_back(s.index_counts[proto]);
+    s.world.insert(s.world.end(), world, world + 16);
+    s.prims.push_back(pi);
+    s.prim_names.push_back(name);
     return (uint32_t)s.prims.size() - 1;
 }
 
 void finalize_scene(lt_host_scene& s, const std::vector<AnalyticLight>& analytic) {
-    // scene bounds (compute_scene_dimensions, LumenScene.cpp:732-751)
+    // scene bounds (compute_scene_dimensions, LumenScene.cpp:732-751): the
+    // meshes' object-space boxes, since the transformed box Bbox::transform
+    // returns (BBox.h:48-61) is discarded there (LumenScene.cpp:735)
     float lo[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f};
     float hi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
     for (auto& p : s.prims)
@@ -299,9 +340,11 @@ void finalize_scene(lt_host_scene& s, const std::vector<AnalyticLight>& analytic
         const lt_material& mat = s.materials[s.prims[m].material_index];
         if (mat.emissive_factor[0] > 0 || mat.emissive_factor[1] > 0 || mat.emissive_factor[2] > 0) {
             lt_light L{};
-            for (int i = 0; i < 4; i++) L.world_matrix[i * 5] = 1.0f;
-            uint32_t end = (m + 1 < n_mesh) ? s.prims[m + 1].index_offset : (uint32_t)s.indices.size();
-            L.num_triangles = (end - s.prims[m].index_offset) / 3;
+            if (s.world.empty())
+                for (int i = 0; i < 4; i++) L.world_matrix[i * 5] = 1.0f;
+            else
+                memcpy(L.world_matrix, &s.world[16 * (size_t)m], 64);  // LumenScene.cpp:86
+            L.num_triangles = mesh_index_count(s, m) / 3;
             L.prim_mesh_idx = m;
             L.light_flags = LT_LIGHT_AREA | LT_LIGHT_FINITE;
             memcpy(L.L, mat.emissive_factor, 12);
@@ -336,11 +379,16 @@ void finalize_scene(lt_host_scene& s, const std::vector<AnalyticLight>& analytic
     for (auto& L : s.lights) {
         if ((L.light_flags & 7u) != LT_LIGHT_AREA) continue;
         const lt_prim_mesh_info& pm = s.prims[L.prim_mesh_idx];
+        const float* W = L.world_matrix;
+        auto world_pos = [&](const float* p, float* o) {  // world_matrix * vec4(p, 1) (LumenScene.cpp:126-128)
+            for (int i = 0; i < 3; i++) o[i] = W[i] * p[0] + W[4 + i] * p[1] + W[8 + i] * p[2] + W[12 + i];
+        };
         for (uint32_t t = 0; t < L.num_triangles; t++) {
             uint32_t io = pm.index_offset + 3 * t;
-            const float* a = s.vertices[s.indices[io] + pm.vertex_offset].pos;
-            const float* b = s.vertices[s.indices[io + 1] + pm.vertex_offset].pos;
-            const float* c = s.vertices[s.indices[io + 2] + pm.vertex_offset].pos;
+            float a[3], b[3], c[3];
+            world_pos(s.vertices[s.indices[io] + pm.vertex_offset].pos, a);
+            world_pos(s.vertices[s.indices[io + 1] + pm.vertex_offset].pos, b);
+            world_pos(s.vertices[s.indices[io + 2] + pm.vertex_offset].pos, c);
             float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
             float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
             area += 0.5f * sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
@@ -693,7 +741,10 @@ static lt_host_scene* make_terrain(const lt_procedural_params& p) {
                 }
             add_buf(*s, "terrain", m, mat);
         }
-    // instanced boxes: uniform random pose, transformed to world space
+    // instanced boxes: one unit cube geometry, one instance (a mesh with its
+    // own material and world matrix T R S: uniform random position on the
+    // terrain, uniform random rotation, random size) per box
+    uint32_t proto = 0;
     for (uint32_t b = 0; b < ninst; b++) {
         s->materials.push_back(rand_mat());
         uint32_t mat = (uint32_t)s->materials.size() - 1;
@@ -704,10 +755,26 @@ static lt_host_scene* make_terrain(const lt_procedural_params& p) {
         float u1 = U(rng), u2 = U(rng), u3 = U(rng);
         float q[4] = {sqrtf(1 - u1) * sinf(6.2831853f * u2), sqrtf(1 - u1) * cosf(6.2831853f * u2),
                       sqrtf(u1) * sinf(6.2831853f * u3), sqrtf(u1) * cosf(6.2831853f * u3)};
-        float h[3] = {sz * 0.5f, sz * 0.5f, sz * 0.5f};
-        MeshBuf m;
-        box(m, c, h, 0.0f, q);
-        add_buf(*s, "box", m, mat);
+        const float qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+        const float R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy)},
+                               {2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx)},
+                               {2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)}};
+        float W[16] = {0};
+        for (int col = 0; col < 3; col++)
+            for (int row = 0; row < 3; row++) W[4 * col + row] = R[row][col] * sz;
+        W[12] = c[0];
+        W[13] = c[1];
+        W[14] = c[2];
+        W[15] = 1.0f;
+        if (b == 0) {  // the cube geometry, as the first box with its own pose
+            MeshBuf m;
+            const float c0[3] = {0.0f, 0.0f, 0.0f}, h[3] = {0.5f, 0.5f, 0.5f};
+            box(m, c0, h, 0.0f);
+            proto = add_mesh(*s, "box", m.p, m.n, m.uv, mat);
+            set_world(*s, proto, W);
+        } else {
+            add_instance(*s, "box", proto, mat, W);
+        }
     }
     // one area light: 16 x 16 quad facing down at y = 45
     lt_material lm = diffuse(1.0f, 1.0f, 1.0f);

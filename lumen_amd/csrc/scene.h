@@ -18,6 +18,10 @@ struct lt_host_scene {
     std::vector<std::vector<float>> tex_data;
     std::vector<lt_texture> textures;
     std::vector<int32_t> rough_tex;  // per material
+    // per prim mesh, empty unless some mesh is an instance: world matrix (16
+    // floats, column-major) and index count (instances share index ranges)
+    std::vector<float> world;
+    std::vector<uint32_t> index_counts;
     uint32_t total_light_triangle_cnt = 0;
     float total_light_area = 0.0f;
     int32_t dir_light_idx = -1;
@@ -42,6 +46,16 @@ struct AnalyticLight {
 // 0..3T-1, as LumenScene.cpp:302-325 lays out OBJ shapes).
 uint32_t add_mesh(lt_host_scene& s, const std::string& name, const std::vector<float>& pos,
                   const std::vector<float>& nrm, const std::vector<float>& uv, uint32_t material);
+
+// Appends an instance of prim mesh `proto`'s geometry (the same index and
+// vertex ranges) with its own material and object-to-world matrix (16 floats,
+// column-major; LumenPrimMesh::world_matrix, LumenScene.h:32).
+uint32_t add_instance(lt_host_scene& s, const std::string& name, uint32_t proto, uint32_t material,
+                      const float world[16]);
+// index count of prim mesh m
+uint32_t mesh_index_count(const lt_host_scene& s, uint32_t m);
+// sets prim mesh m's world matrix (16 floats, column-major)
+void set_world(lt_host_scene& s, uint32_t m, const float world[16]);
 
 // Lights, light triangle count, light area and scene bounds exactly as
 // LumenScene::load_scene:70-129 and compute_scene_dimensions:732-751.

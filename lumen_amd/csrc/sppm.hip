@@ -225,9 +225,11 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     if (COUNT) flush_counts(c, counts);
 }
 
-// max.comp / min.comp + reduce passes: one atomic per wave and value
+// max.comp / min.comp + reduce passes: a wave reduction, a workgroup
+// reduction in LDS, then one atomic per workgroup and value
 __global__ __launch_bounds__(256) void k_sppm_bounds(const float* __restrict__ st, uint32_t n,
                                                      SppmBounds* __restrict__ b) {
+    __shared__ float red[4][7];
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, mr = -INFINITY;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
@@ -249,8 +251,23 @@ __global__ __launch_bounds__(256) void k_sppm_bounds(const float* __restrict__ s
         }
         mr = fmaxf(mr, __shfl_xor(mr, o));
     }
+    const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0u) {
-#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            red[w][k] = lo[k];
+            red[w][3 + k] = hi[k];
+        }
+        red[w][6] = mr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t v = 1; v < blockDim.x / 64u; v++) {
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fminf(lo[k], red[v][k]);
+                hi[k] = fmaxf(hi[k], red[v][3 + k]);
+            }
+            mr = fmaxf(mr, red[v][6]);
+        }
         for (int k = 0; k < 3; k++) {
             atomicMin(&b->mn[k], ord_enc(lo[k]));
             atomicMax(&b->mx[k], ord_enc(hi[k]));
@@ -537,8 +554,8 @@ lt_status launch_sppm(lt_context* ctx, SppmBuffers& b, const SppmLaunch& L0, uin
             hipLaunchKernelGGL((k_sppm_eye<false>), dim3(eye_blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L,
                                b.state, ctx->counters, ctx->spill, cnt);
         if (f == 0) phase_mark(ctx);
-        hipLaunchKernelGGL(k_sppm_bounds, dim3((unsigned)std::min<size_t>((N + 255) / 256, 2048)), dim3(256), 0,
-                           ctx->stream, b.state, (uint32_t)N, b.bounds);
+        hipLaunchKernelGGL(k_sppm_bounds, dim3((unsigned)std::min<size_t>((N + 255) / 256, 4u * ctx->num_cus)),
+                           dim3(256), 0, ctx->stream, b.state, (uint32_t)N, b.bounds);
         if (f == 0) phase_mark(ctx);
         if (count)
             hipLaunchKernelGGL((k_sppm_light<true>), dim3(light_blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L,

@@ -135,6 +135,16 @@ struct Tri {
     vec3 v0, v1, v2;
     uint32_t prim, inst, gid;
 };
+// Instance transforms (lt_scene_desc::world_matrices, LumenScene.h:32).  The
+// record of a column-major 4x4 matrix: object-to-world rows (3 x 4), then the
+// rows of the normal matrix, the inverse transpose of the linear part, from
+// its cofactors in double precision.  xf_record returns false for the
+// identity (and for a singular matrix, which the upload refuses).
+bool xf_record(const f32* m, f32 out[24]);
+// mesh m's transform record, false if it has none (identity)
+bool mesh_xf(const lt_scene_desc& s, uint32_t m, f32 out[24]);
+// mesh m's index range (index_counts, or up to the next mesh's offset)
+void mesh_range(const lt_scene_desc& s, uint32_t m, uint32_t& off, uint32_t& cnt);
 struct Accel;  // host BVH2 over Tri
 Accel* accel_build(const lt_scene_desc& s);
 void accel_free(Accel* a);

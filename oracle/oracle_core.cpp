@@ -892,14 +892,20 @@ TriangleRecord sample_triangle(const lt_scene_desc& s, const lt_prim_mesh_info& 
     float sr = sqrtf(rands.x);
     uv = {1.0f - sr, rands.y * sr};
     vec3 bc = {1.0f - uv.x - uv.y, uv.x, uv.y};
-    // world matrices of lights are the identity on every path this core takes
-    // (vertices arrive in world space); the transform is kept for the pdf
-    vec4 e0 = mul(world, vec4{v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 1.0f});
-    vec4 e1 = mul(world, vec4{v2p.x - v0.x, v2p.y - v0.y, v2p.z - v0.z, 1.0f});
+    // The reference transforms the edges with w = 1 (commons.glsl:136-137),
+    // which only matters for a translated light; they are vectors (w = 0)
+    // here, so the pdf is the world triangle's.  The normal goes through
+    // transpose(inverse(world)) (commons.glsl:130,144).
+    vec4 e0 = mul(world, vec4{v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f});
+    vec4 e1 = mul(world, vec4{v2p.x - v0.x, v2p.y - v0.y, v2p.z - v0.z, 0.0f});
     vec3 pos = v0 * bc.x + v1 * bc.y + v2p * bc.z;
     vec3 nrm = normalize(n0 * bc.x + n1 * bc.y + n2 * bc.z);
     vec4 wp = mul(world, vec4{pos.x, pos.y, pos.z, 1.0f});
-    r.n_s = normalize(nrm);  // inverse-transpose of the identity
+    f32 xf[24];
+    if (xf_record(world, xf))
+        nrm = v3(xf[12] * nrm.x + xf[13] * nrm.y + xf[14] * nrm.z, xf[15] * nrm.x + xf[16] * nrm.y + xf[17] * nrm.z,
+                 xf[18] * nrm.x + xf[19] * nrm.y + xf[20] * nrm.z);
+    r.n_s = normalize(nrm);
     r.triangle_pdf = 2.0f / length(cross(v3(e0.x, e0.y, e0.z), v3(e1.x, e1.y, e1.z)));
     r.pos = v3(wp.x, wp.y, wp.z);
     return r;

@@ -10,6 +10,7 @@
 // with the triangle test of Woop, Benthin & Wald, "Watertight Ray/Triangle
 // Intersection", JCGT 2(1), 2013 (including its double-precision edge fallback).
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "oracle.h"
@@ -69,20 +70,74 @@ bool intersect_tri(const Tri& tr, vec3 org, vec3 dir, float tmin, float tmax, fl
     return true;
 }
 
+bool xf_record(const f32* m, f32 out[24]) {
+    bool identity = true;
+    for (int k = 0; k < 16; k++)
+        if (m[k] != ((k % 5 == 0) ? 1.0f : 0.0f)) identity = false;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) out[4 * i + j] = m[j * 4 + i];
+    double a[3][3], c[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = m[j * 4 + i];
+    c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    c[0][1] = -(a[1][0] * a[2][2] - a[1][2] * a[2][0]);
+    c[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    c[1][0] = -(a[0][1] * a[2][2] - a[0][2] * a[2][1]);
+    c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+    c[1][2] = -(a[0][0] * a[2][1] - a[0][1] * a[2][0]);
+    c[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+    c[2][1] = -(a[0][0] * a[1][2] - a[0][2] * a[1][0]);
+    c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+    const double det = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
+    if (!(std::fabs(det) > 1e-30)) return false;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[12 + 3 * i + j] = (f32)(c[i][j] / det);
+    out[21] = out[22] = out[23] = 0.0f;
+    return !identity;
+}
+
+bool mesh_xf(const lt_scene_desc& s, uint32_t m, f32 out[24]) {
+    return s.world_matrices && xf_record(s.world_matrices + 16 * (size_t)m, out);
+}
+
+void mesh_range(const lt_scene_desc& s, uint32_t m, uint32_t& off, uint32_t& cnt) {
+    off = s.prim_infos[m].index_offset;
+    cnt = s.index_counts ? s.index_counts[m]
+                         : ((m + 1 < s.num_prim_infos) ? s.prim_infos[m + 1].index_offset : s.num_indices) - off;
+}
+
+// world-space point through the rows of a transform record, in the order
+// ((m0 x + m1 y) + m2 z) + m3 (the device builder's order)
+static vec3 xf_point_rows(const f32* m, vec3 p) {
+    return {((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+            ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]};
+}
+
+// The flattened triangle list: mesh by mesh (instances of one geometry each
+// with their own triangles), vertices in world space; gid = position in the
+// list, the tie-break key of equal hit distances.
 static void collect_tris(const lt_scene_desc& s, std::vector<Tri>& out) {
+    uint32_t gid = 0;
     for (uint32_t m = 0; m < s.num_prim_infos; m++) {
         const lt_prim_mesh_info& pi = s.prim_infos[m];
-        uint32_t end = (m + 1 < s.num_prim_infos) ? s.prim_infos[m + 1].index_offset : s.num_indices;
-        uint32_t ntri = (end - pi.index_offset) / 3;
-        for (uint32_t t = 0; t < ntri; t++) {
-            uint32_t io = pi.index_offset + 3 * t;
+        uint32_t off, cnt;
+        mesh_range(s, m, off, cnt);
+        f32 xf[24];
+        const bool has_xf = mesh_xf(s, m, xf);
+        for (uint32_t t = 0; t < cnt / 3; t++) {
+            uint32_t io = off + 3 * t;
             Tri tr;
             tr.v0 = load3(s.vertices[s.indices[io] + pi.vertex_offset].pos);
             tr.v1 = load3(s.vertices[s.indices[io + 1] + pi.vertex_offset].pos);
             tr.v2 = load3(s.vertices[s.indices[io + 2] + pi.vertex_offset].pos);
+            if (has_xf) {
+                tr.v0 = xf_point_rows(xf, tr.v0);
+                tr.v1 = xf_point_rows(xf, tr.v1);
+                tr.v2 = xf_point_rows(xf, tr.v2);
+            }
             tr.prim = t;
             tr.inst = m;
-            tr.gid = io / 3;
+            tr.gid = gid++;
             out.push_back(tr);
         }
     }
@@ -262,8 +317,11 @@ bool brute_any(const Accel* a, vec3 o, vec3 d, float tmin, float tmax) {
     return false;
 }
 
-// ray.rchit:24-84 with identity object<->world transforms (the JSON path,
-// LumenScene.cpp:328).  dist = gl_RayTminEXT + gl_HitTEXT (ray.rchit:81).
+// ray.rchit:24-84: object-space interpolation, then the instance's
+// object-to-world transform (gl_ObjectToWorldEXT) for the position and the
+// inverse transpose (nrm * gl_WorldToObjectEXT) for the normals; identity for
+// meshes without a transform (the JSON path, LumenScene.cpp:328).
+// dist = gl_RayTminEXT + gl_HitTEXT (ray.rchit:81).
 void fill_payload(const lt_scene_desc& s, const Hit& h, vec3 o, vec3 d, float tmin, Payload& p) {
     (void)o;
     (void)d;
@@ -280,14 +338,30 @@ void fill_payload(const lt_scene_desc& s, const Hit& h, vec3 o, vec3 d, float tm
     vec3 nrm = normalize(n0 * bc.x + n1 * bc.y + n2 * bc.z);
     vec2 uv = {a.uv0[0] * bc.x + b.uv0[0] * bc.y + c.uv0[0] * bc.z, a.uv0[1] * bc.x + b.uv0[1] * bc.y + c.uv0[1] * bc.z};
     vec3 e0 = v2p - v0, e1 = v1 - v0;
-    p.n_g = normalize(cross(e0, e1));
-    p.n_s = normalize(nrm);
-    p.pos = pos;
+    f32 xf[24];
+    if (mesh_xf(s, h.inst, xf)) {
+        auto nrm_xf = [&](vec3 n) {
+            return vec3{xf[12] * n.x + xf[13] * n.y + xf[14] * n.z, xf[15] * n.x + xf[16] * n.y + xf[17] * n.z,
+                        xf[18] * n.x + xf[19] * n.y + xf[20] * n.z};
+        };
+        auto vec_xf = [&](vec3 d) {
+            return vec3{xf[0] * d.x + xf[1] * d.y + xf[2] * d.z, xf[4] * d.x + xf[5] * d.y + xf[6] * d.z,
+                        xf[8] * d.x + xf[9] * d.y + xf[10] * d.z};
+        };
+        p.n_g = normalize(nrm_xf(cross(e0, e1)));
+        p.n_s = normalize(nrm_xf(nrm));
+        p.pos = vec_xf(pos) + v3(xf[3], xf[7], xf[11]);
+        p.area = 0.5f * length(cross(vec_xf(e0), vec_xf(e1)));
+    } else {
+        p.n_g = normalize(cross(e0, e1));
+        p.n_s = normalize(nrm);
+        p.pos = pos;
+        p.area = 0.5f * length(cross(e0, e1));
+    }
     p.uv = uv;
     p.material_idx = pi.material_index;
     p.triangle_idx = h.prim;
     p.instance_idx = h.inst;
-    p.area = 0.5f * length(cross(e0, e1));
     p.dist = tmin + h.t;
 }
 

@@ -18,6 +18,7 @@ class DescScene:
         self.meshes = []  # (positions (T,3,3), normals (T,3,3) or None, material)
         self.materials = []
         self.analytic = []  # dict(type, pos, to, L)
+        self.instances = []  # (mesh index of the geometry, material, 4x4 object-to-world)
 
     def material(self, **kw) -> int:
         m = _abi.lt_material()
@@ -38,9 +39,15 @@ class DescScene:
             normals = np.repeat(n[:, None, :], 3, axis=1)
         self.meshes.append((tris, np.asarray(normals, np.float32).reshape(-1, 3, 3), mat))
 
+    def instance(self, mesh, mat, world):
+        """Another mesh that shares mesh `mesh`'s geometry, with its own
+        material and object-to-world matrix (4 x 4, row-major here)."""
+        self.instances.append((mesh, mat, np.asarray(world, np.float64)))
+
     def build(self):
         verts, idx, prims, lights = [], [], [], []
         n_light_tris = 0
+        counts, worlds = [], []
         for mi, (tris, nrm, mat) in enumerate(self.meshes):
             pi = _abi.lt_prim_mesh_info()
             pi.index_offset = len(idx)
@@ -56,17 +63,27 @@ class DescScene:
                     idx.append(3 * t + k)
                     verts.append(v)
             prims.append(pi)
-            m = self.materials[mat]
+            counts.append(3 * len(tris))
+            worlds.append(np.eye(4))
+        for mesh, mat, W in self.instances:
+            src = prims[mesh]
+            pi = _abi.lt_prim_mesh_info()
+            pi.index_offset, pi.vertex_offset, pi.material_index = src.index_offset, src.vertex_offset, mat
+            pi.min_pos[:3], pi.max_pos[:3] = list(src.min_pos[:3]), list(src.max_pos[:3])
+            prims.append(pi)
+            counts.append(counts[mesh])
+            worlds.append(W)
+        for mi, pi in enumerate(prims):
+            m = self.materials[pi.material_index]
             if any(m.emissive_factor[i] > 0 for i in range(3)):
                 L = _abi.lt_light()
-                for i in range(4):
-                    L.world_matrix[5 * i] = 1.0
+                L.world_matrix[:] = list(worlds[mi].T.reshape(-1).astype(np.float32))  # column-major
                 L.prim_mesh_idx = mi
-                L.num_triangles = len(tris)
+                L.num_triangles = counts[mi] // 3
                 L.light_flags = 2 | 16
                 L.L[:] = list(m.emissive_factor)
                 lights.append(L)
-                n_light_tris += len(tris)
+                n_light_tris += counts[mi] // 3
         allp = np.concatenate([t.reshape(-1, 3) for t, _, _ in self.meshes])
         lo, hi = allp.min(0), allp.max(0)
         for a in self.analytic:
@@ -94,6 +111,12 @@ class DescScene:
         d.dir_light_idx = -1
         d.scene_min[:] = list(lo)
         d.scene_max[:] = list(hi)
+        if self.instances:
+            self._w = (C.c_float * (16 * len(prims)))(*np.concatenate(
+                [W.T.reshape(-1) for W in worlds]).astype(np.float32))
+            self._c = (C.c_uint32 * len(prims))(*counts)
+            d.world_matrices = C.cast(self._w, C.POINTER(C.c_float))
+            d.index_counts = C.cast(self._c, C.POINTER(C.c_uint32))
         self.desc = d
         return d
 

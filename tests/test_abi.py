@@ -47,7 +47,8 @@ def test_config_structs_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = ["lt_path_config", "lt_bdpt_config", "lt_vcm_config", "lt_sppm_config", "lt_frame_params",
-               "lt_render_stats", "lt_scene_integrator_cfg", "lt_adjoint_config", "lt_camera"]
+               "lt_render_stats", "lt_scene_integrator_cfg", "lt_adjoint_config", "lt_camera", "lt_scene_desc",
+               "lt_tuning"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for name in structs:
         lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
@@ -143,3 +144,6 @@ def test_terrain_workload_size():
     assert d.num_lights == 1 + 4
     assert d.total_light_triangle_cnt == 2 + 4  # each analytic light counts one (LumenScene.cpp:106)
     assert sc.num_triangles == 2 * 100 * 100 + 10 * 12 + 2
+    # the boxes are instances of one 12-triangle cube: its geometry is stored once
+    assert d.num_indices // 3 == 2 * 100 * 100 + 12 + 2
+    assert d.num_prim_infos == 16 + 10 + 1 and bool(d.world_matrices) and bool(d.index_counts)
