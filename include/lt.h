@@ -216,12 +216,23 @@ lt_status lt_device_info(lt_context* ctx, int* num_cus, char* arch, size_t arch_
 
 /* Replaces LumenScene::load_scene's buffer uploads (LumenScene.cpp:146-216). */
 lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* scene);
+/* The host-side checks of lt_scene_upload without a device (index ranges,
+ * references, transforms); err receives the reason of a refusal.  No
+ * reference counterpart. */
+lt_status lt_scene_validate(const lt_scene_desc* scene, char* err, size_t err_len);
 
 /* Build options for lt_accel_build. */
 typedef struct lt_build_opts {
     uint32_t max_leaf_tris; /* triangles per BVH8 leaf child, 1..4 (0: the default, 1) */
-    uint32_t reserved[7];
+    uint32_t collapse;      /* LT_COLLAPSE_SAH (0: the default) or LT_COLLAPSE_GREEDY */
+    uint32_t reserved[6];
 } lt_build_opts;
+/* Collapse of the binary LBVH into BVH8 nodes: the SAH-optimal choice of each
+ * node's children by dynamic programming over the binary tree (Ylitie, Karras
+ * and Laine 2017, node cost 1, triangle cost 0.5), or the greedy expansion of
+ * the child with the largest surface area. */
+#define LT_COLLAPSE_SAH 1u
+#define LT_COLLAPSE_GREEDY 2u
 
 /* Build statistics. */
 typedef struct lt_build_stats {

@@ -146,9 +146,10 @@ class Context:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
         self.scene = scene
 
-    def build_accel(self, max_leaf_tris: int = 1) -> _abi.lt_build_stats:
+    def build_accel(self, max_leaf_tris: int = 1, collapse: str = "sah") -> _abi.lt_build_stats:
         o = _abi.lt_build_opts()
         o.max_leaf_tris = max_leaf_tris
+        o.collapse = {"sah": _abi.LT_COLLAPSE_SAH, "greedy": _abi.LT_COLLAPSE_GREEDY}[collapse]
         s = _abi.lt_build_stats()
         check(lib().lt_accel_build(self._h, C.byref(o), C.byref(s)), self._h, "lt_accel_build")
         return s

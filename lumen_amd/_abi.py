@@ -33,6 +33,7 @@ LT_PATH_MEGAKERNEL, LT_PATH_WAVEFRONT, LT_PATH_PERSISTENT = 0, 1, 2
 LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
 LT_WF_SORT_NONE, LT_WF_SORT_BSDF = 1, 2
+LT_COLLAPSE_SAH, LT_COLLAPSE_GREEDY = 1, 2
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
 
@@ -84,7 +85,7 @@ class lt_camera(C.Structure):
 
 
 class lt_build_opts(C.Structure):
-    _fields_ = [("max_leaf_tris", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+    _fields_ = [("max_leaf_tris", C.c_uint32), ("collapse", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 class lt_build_stats(C.Structure):
@@ -162,6 +163,7 @@ _PROTOS = {
     "lt_last_error": (C.c_char_p, [P]),
     "lt_device_info": (C.c_int, [P, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "lt_scene_upload": (C.c_int, [P, C.POINTER(lt_scene_desc)]),
+    "lt_scene_validate": (C.c_int, [C.POINTER(lt_scene_desc), C.c_char_p, C.c_size_t]),
     "lt_accel_build": (C.c_int, [P, C.POINTER(lt_build_opts), C.POINTER(lt_build_stats)]),
     "lt_accel_export": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),
     "lt_trace_closest": (C.c_int, [P, P, C.c_size_t, P, C.c_uint32]),

@@ -8,10 +8,14 @@
 //   3. radix sort      : (morton, triangle) pairs, rocPRIM radix sort (bits 0..29)
 //   4. k_karras        : binary radix tree (Karras 2012), one thread per node
 //   5. k_refit         : bottom-up AABBs with arrival counters
+//   5'. (in k_refit)    : SAH dynamic programme over the binary tree (Ylitie,
+//                        Karras & Laine 2017): per node the cheapest cover of
+//                        its subtree by at most i = 1..8 BVH8 subtree roots;
 //   6. k_collapse      : top-down BVH8 collapse, one thread per BVH8 node and
-//                        level; greedy expansion of the child with the largest
-//                        surface area, subtrees of <= max_leaf triangles become
-//                        leaves; 16-bit quantised, 128-B nodes.
+//                        level; a node's children are the roots the programme
+//                        chose (or, greedy, the expansion of the child with the
+//                        largest surface area), subtrees of <= max_leaf
+//                        triangles become leaves; 16-bit quantised, 128-B nodes.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -211,11 +215,85 @@ __device__ __forceinline__ Box tri_box(const TriRec& r) {
     return b;
 }
 
-// bottom-up refit: second arrival at a node computes its box
+__device__ __forceinline__ float half_area(const Box& b) {
+    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// SAH weights of a BVH8 node visit and of a triangle test (a watertight
+// test costs about half a node step)
+constexpr float kSahNode = 1.0f;
+constexpr float kSahTri = 0.5f;
+constexpr uint8_t kDecLeaf = 0xFF;
+
+// The dynamic programme at internal node p (children a, b):
+//   R(p, i) = cheapest cover of p's subtree by at most i subtree roots,
+//   R(p, 1) = min(leaf cost (<= max_leaf triangles), A(p) kSahNode + min_k R(a, k) + R(b, 8 - k)),
+//   R(p, i) = min(R(p, i - 1), min_k R(a, k) + R(b, i - k)),  i = 2..8,
+// a binary leaf costing A kSahTri for any i.  dec[p][0] = kDecLeaf or the k of
+// the node's own 8 roots; dec[p][i - 1] = 0 (fewer roots) or the k of the split.
+__device__ void sah_node(uint32_t p, uint32_t a, uint32_t b, const Box& pb, uint32_t cnt, uint32_t max_leaf,
+                         const volatile Box* lbox, const volatile float* R_all, float* __restrict__ R,
+                         uint8_t* __restrict__ dec) {
+    float ra[8], rb[8];
+    auto load = [&](uint32_t c, float* r) {
+        if (c & 0x80000000u) {
+            const volatile Box& bx = lbox[c & 0x7fffffffu];
+            Box q;
+            for (int k = 0; k < 3; k++) {
+                q.lo[k] = bx.lo[k];
+                q.hi[k] = bx.hi[k];
+            }
+            const float v = half_area(q) * kSahTri;
+            for (int i = 0; i < 8; i++) r[i] = v;
+        } else {
+            for (int i = 0; i < 8; i++) r[i] = R_all[(size_t)c * 8 + i];
+        }
+    };
+    load(a, ra);
+    load(b, rb);
+    float out[8];
+    uint8_t d[8];
+    float node = INFINITY;
+    uint8_t node_k = 1;
+    for (int k = 1; k < 8; k++) {
+        const float v = ra[k - 1] + rb[8 - k - 1];
+        if (v < node) {
+            node = v;
+            node_k = (uint8_t)k;
+        }
+    }
+    node += half_area(pb) * kSahNode;
+    const float leaf = cnt <= max_leaf ? half_area(pb) * kSahTri * (float)cnt : INFINITY;
+    out[0] = leaf <= node ? leaf : node;
+    d[0] = leaf <= node ? kDecLeaf : node_k;
+    for (int i = 2; i <= 8; i++) {
+        float best = out[i - 2];
+        uint8_t bk = 0;
+        for (int k = 1; k < i; k++) {
+            const float v = ra[k - 1] + rb[i - k - 1];
+            if (v < best) {
+                best = v;
+                bk = (uint8_t)k;
+            }
+        }
+        out[i - 1] = best;
+        d[i - 1] = bk;
+    }
+    for (int i = 0; i < 8; i++) {
+        R[(size_t)p * 8 + i] = out[i];
+        dec[(size_t)p * 8 + i] = d[i];
+    }
+}
+
+// bottom-up refit: second arrival at a node computes its box (and, when R is
+// given, the SAH programme of the node)
 __global__ void k_refit(const TriRec* __restrict__ tris, const uint32_t* __restrict__ sorted, int n,
                         const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
                         const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
-                        uint32_t* __restrict__ flags, Box* __restrict__ ibox, Box* __restrict__ lbox) {
+                        uint32_t* __restrict__ flags, Box* __restrict__ ibox, Box* __restrict__ lbox,
+                        const uint32_t* __restrict__ rlo, const uint32_t* __restrict__ rhi, uint32_t max_leaf,
+                        float* __restrict__ R, uint8_t* __restrict__ dec) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     lbox[i] = tri_box(tris[sorted[i]]);
@@ -238,6 +316,7 @@ __global__ void k_refit(const TriRec* __restrict__ tris, const uint32_t* __restr
             r.hi[k] = fmaxf(ba->hi[k], bb->hi[k]);
         }
         ibox[p] = r;
+        if (R) sah_node(p, a, b, r, rhi[p] - rlo[p] + 1, max_leaf, lbox, R, R, dec);
         __threadfence();
         if (p == 0) return;
         p = parent_int[p];
@@ -248,11 +327,6 @@ struct Task {
     uint32_t bnode;  // binary node (bit 31: leaf)
     uint32_t slot;   // BVH8 node index
 };
-
-__device__ __forceinline__ float half_area(const Box& b) {
-    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
-    return dx * dy + dy * dz + dz * dx;
-}
 
 __device__ __forceinline__ uint32_t count_of(uint32_t c, const uint32_t* rlo, const uint32_t* rhi) {
     if (c & 0x80000000u) return 1;
@@ -269,7 +343,8 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
                            const uint32_t* __restrict__ rlo, const uint32_t* __restrict__ rhi,
                            const Box* __restrict__ ibox, const Box* __restrict__ lbox,
                            const uint32_t* __restrict__ sorted, const TriRec* __restrict__ tris_in,
-                           TriRec* __restrict__ tris_out, Node8* __restrict__ nodes, uint32_t max_leaf) {
+                           TriRec* __restrict__ tris_out, Node8* __restrict__ nodes, uint32_t max_leaf,
+                           const uint8_t* __restrict__ dec) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_in) return;
     const Task task = in[g];
@@ -277,8 +352,36 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
     int nc = 0;
     if (task.bnode & 0x80000000u) {
         ch[nc++] = task.bnode;  // a single-triangle scene
-    } else if (count_of(task.bnode, rlo, rhi) <= max_leaf) {
+    } else if (count_of(task.bnode, rlo, rhi) <= max_leaf && (!dec || dec[(size_t)task.bnode * 8] == kDecLeaf)) {
         ch[nc++] = task.bnode;  // root small enough to be one leaf
+    } else if (dec) {
+        // the roots the SAH programme chose: the node's own split into 8,
+        // then each (subtree, i roots) pair resolved by its decision
+        uint32_t st_m[8], st_i[8];
+        int sp = 0;
+        const uint32_t k0 = dec[(size_t)task.bnode * 8] == kDecLeaf ? 4u : dec[(size_t)task.bnode * 8];
+        st_m[sp] = left[task.bnode];
+        st_i[sp++] = k0;
+        st_m[sp] = right[task.bnode];
+        st_i[sp++] = 8u - k0;
+        while (sp > 0) {
+            sp--;
+            const uint32_t m = st_m[sp], i = st_i[sp];
+            if ((m & 0x80000000u) || i == 1u) {
+                ch[nc++] = m;
+                continue;
+            }
+            const uint32_t dk = dec[(size_t)m * 8 + (i - 1)];
+            if (dk == 0u) {
+                st_m[sp] = m;
+                st_i[sp++] = i - 1u;
+            } else {
+                st_m[sp] = left[m];
+                st_i[sp++] = dk;
+                st_m[sp] = right[m];
+                st_i[sp++] = i - dk;
+            }
+        }
     } else {
         ch[nc++] = left[task.bnode];
         ch[nc++] = right[task.bnode];
@@ -305,21 +408,25 @@ __global__ void k_collapse(const Task* __restrict__ in, uint32_t n_in, Task* __r
     uint32_t n_int = 0, n_leaf_tris = 0;
     bool is_leaf[8];
     {
+        // a child is a leaf if it is a binary leaf, or holds at most max_leaf
+        // triangles (and the SAH programme made it a leaf)
+        auto leaf_child = [&](uint32_t c) {
+            if (c & 0x80000000u) return true;
+            if (count_of(c, rlo, rhi) > max_leaf) return false;
+            return !dec || dec[(size_t)c * 8] == kDecLeaf || nc == 1;
+        };
         uint32_t ord[8];
         int ni = 0, nl = 0;
         for (int c = 0; c < nc; c++) {
-            const uint32_t cnt = count_of(ch[c], rlo, rhi);
-            if ((ch[c] & 0x80000000u) || cnt <= max_leaf) {
-                n_leaf_tris += cnt;
+            if (leaf_child(ch[c])) {
+                n_leaf_tris += count_of(ch[c], rlo, rhi);
             } else {
                 ord[ni++] = ch[c];
                 n_int++;
             }
         }
-        for (int c = 0; c < nc; c++) {
-            const uint32_t cnt = count_of(ch[c], rlo, rhi);
-            if ((ch[c] & 0x80000000u) || cnt <= max_leaf) ord[ni + nl++] = ch[c];
-        }
+        for (int c = 0; c < nc; c++)
+            if (leaf_child(ch[c])) ord[ni + nl++] = ch[c];
         for (int c = 0; c < nc; c++) {
             ch[c] = ord[c];
             is_leaf[c] = c >= (int)n_int;
@@ -406,6 +513,10 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
     if (!ctx->scene.uploaded) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_accel_build: no scene uploaded");
     uint32_t max_leaf = opts && opts->max_leaf_tris ? opts->max_leaf_tris : 1u;
     if (max_leaf < 1 || max_leaf > 4) return lt_fail(ctx, LT_ERR_INVALID_ARG, "max_leaf_tris must be in 1..4");
+    const uint32_t collapse = opts && opts->collapse ? opts->collapse : LT_COLLAPSE_SAH;
+    if (collapse != LT_COLLAPSE_SAH && collapse != LT_COLLAPSE_GREEDY)
+        return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown collapse");
+    const bool sah = collapse == LT_COLLAPSE_SAH;
     hipStream_t st = ctx->stream;
     DeviceBVH& bvh = ctx->bvh;
     if (bvh.nodes) (void)hipFree(bvh.nodes);
@@ -427,6 +538,8 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
     Task *q0 = nullptr, *q1 = nullptr;
     void* sort_tmp = nullptr;
     size_t sort_bytes = 0;
+    float* sah_R = nullptr;
+    uint8_t* sah_dec = nullptr;
     lt_status rc = LT_OK;
     auto fail = [&](hipError_t e, const char* what) {
         rc = lt_hip_check(ctx, e, what);
@@ -480,8 +593,12 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
                                pleaf, rlo, rhi);
             BCHK(hipGetLastError());
         }
+        if (sah && n > 1) {
+            BCHK(hipMalloc(&sah_R, sizeof(float) * 8 * (size_t)ni));
+            BCHK(hipMalloc(&sah_dec, 8 * (size_t)ni));
+        }
         hipLaunchKernelGGL(k_refit, dim3(G), dim3(B), 0, st, tris_unsorted, vals2, (int)n, left, right, pint, pleaf,
-                           flags, ibox, lbox);
+                           flags, ibox, lbox, rlo, rhi, max_leaf, sah_R, sah_dec);
         BCHK(hipGetLastError());
         // top-down collapse, one launch per BVH8 level
         uint32_t h_ctr[4] = {1u, 0u, 0u, 0u};  // node 0 is the root
@@ -493,7 +610,7 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
             depth++;
             BCHK(hipMemsetAsync(ctr + 2, 0, 4, st));
             hipLaunchKernelGGL(k_collapse, dim3((n_in + 63) / 64), dim3(64), 0, st, q0, n_in, q1, ctr, left, right,
-                               rlo, rhi, ibox, lbox, vals2, tris_unsorted, bvh.tris, bvh.nodes, max_leaf);
+                               rlo, rhi, ibox, lbox, vals2, tris_unsorted, bvh.tris, bvh.nodes, max_leaf, sah_dec);
             BCHK(hipGetLastError());
             BCHK(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
             BCHK(hipStreamSynchronize(st));
@@ -560,7 +677,7 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
 cleanup:
 #undef BCHK
     void* tmp[] = {tris_unsorted, cb, keys, vals, keys2, vals2, left, right, pint, pleaf, rlo, rhi, flags, ibox,
-                   lbox, q0, q1, ctr, sort_tmp};
+                   lbox, q0, q1, ctr, sort_tmp, sah_R, sah_dec};
     for (void* p : tmp)
         if (p) (void)hipFree(p);
     if (rc != LT_OK) {

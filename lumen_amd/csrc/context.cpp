@@ -199,70 +199,103 @@ lt_status lt_device_info(lt_context* ctx, int* num_cus, char* arch, size_t arch_
     return LT_OK;
 }
 
-lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
-    if (!ctx || !s) return LT_ERR_INVALID_ARG;
-    if (s->num_indices % 3 != 0) return lt_fail(ctx, LT_ERR_INVALID_ARG, "num_indices is not a multiple of 3");
+// What lt_scene_upload derives from a valid scene on the host.
+struct SceneHost {
+    std::vector<uint32_t> tri_start;  // first flattened triangle of each mesh (+ the total)
+    std::vector<float> xf, lxf;       // per mesh / per light transform records
+    bool any_xf = false, any_lxf = false;
+};
+
+// Host-side validation: kernels assume every index and reference in range.
+static lt_status validate_scene(const lt_scene_desc* s, SceneHost& h, std::string& err) {
+    auto bad = [&](lt_status st, const std::string& m) {
+        err = m;
+        return st;
+    };
+    if (s->num_indices % 3 != 0) return bad(LT_ERR_INVALID_ARG, "num_indices is not a multiple of 3");
     if ((s->num_indices && (!s->indices || !s->vertices || !s->prim_infos)) || (s->num_materials && !s->materials) ||
         (s->num_lights && !s->lights) || (s->num_textures && !s->textures))
-        return lt_fail(ctx, LT_ERR_INVALID_ARG, "scene array pointer is NULL");
-    // validate indices and references on the host: kernels assume them in range
-    std::vector<uint32_t> tri_start(s->num_prim_infos + 1, 0u);
+        return bad(LT_ERR_INVALID_ARG, "scene array pointer is NULL");
+    if (s->num_indices && s->num_prim_infos == 0) return bad(LT_ERR_INVALID_ARG, "indices without meshes");
+    h.tri_start.assign(s->num_prim_infos + 1, 0u);
     std::set<std::tuple<uint64_t, uint64_t, uint32_t>> checked;  // geometry shared by instances: once
     for (uint32_t m = 0; m < s->num_prim_infos; m++) {
         const lt_prim_mesh_info& pi = s->prim_infos[m];
         uint64_t off, cnt;
         if (!mesh_range(s, m, off, cnt) || pi.material_index >= s->num_materials)
-            return lt_fail(ctx, LT_ERR_INVALID_ARG, "prim_infos[" + std::to_string(m) + "] out of range");
-        if ((uint64_t)tri_start[m] + cnt / 3 >= (1ull << 31))
-            return lt_fail(ctx, LT_ERR_UNSUPPORTED, "more than 2^31 triangles over all meshes");
-        tri_start[m + 1] = tri_start[m] + (uint32_t)(cnt / 3);
+            return bad(LT_ERR_INVALID_ARG, "prim_infos[" + std::to_string(m) + "] out of range");
+        if ((uint64_t)h.tri_start[m] + cnt / 3 >= (1ull << 31))
+            return bad(LT_ERR_UNSUPPORTED, "more than 2^31 triangles over all meshes");
+        h.tri_start[m + 1] = h.tri_start[m] + (uint32_t)(cnt / 3);
         if (!checked.insert({off, cnt, pi.vertex_offset}).second) continue;
         for (uint64_t i = off; i < off + cnt; i++)
             if ((uint64_t)s->indices[i] + pi.vertex_offset >= s->num_vertices)
-                return lt_fail(ctx, LT_ERR_INVALID_ARG, "index " + std::to_string(i) + " out of range");
+                return bad(LT_ERR_INVALID_ARG, "index " + std::to_string(i) + " out of range");
     }
     if (!s->index_counts && s->num_prim_infos && s->prim_infos[0].index_offset != 0)
-        return lt_fail(ctx, LT_ERR_INVALID_ARG, "prim_infos[0].index_offset must be 0");
-    // transforms: uploaded only where some mesh or light is not the identity
-    std::vector<float> xf, lxf;
-    bool any_xf = false, any_lxf = false;
+        return bad(LT_ERR_INVALID_ARG, "prim_infos[0].index_offset must be 0");
+    // transforms: uploaded only where some mesh or area light is not the identity
     if (s->world_matrices) {
-        xf.resize((size_t)kXfFloats * s->num_prim_infos);
+        h.xf.resize((size_t)kXfFloats * s->num_prim_infos);
         for (uint32_t m = 0; m < s->num_prim_infos; m++) {
             bool id;
-            if (!make_xf(s->world_matrices + 16 * (size_t)m, xf.data() + (size_t)kXfFloats * m, id))
-                return lt_fail(ctx, LT_ERR_INVALID_ARG, "world_matrices[" + std::to_string(m) + "] is singular or not finite");
-            any_xf |= !id;
+            if (!make_xf(s->world_matrices + 16 * (size_t)m, h.xf.data() + (size_t)kXfFloats * m, id))
+                return bad(LT_ERR_INVALID_ARG, "world_matrices[" + std::to_string(m) + "] is singular or not finite");
+            h.any_xf |= !id;
         }
     }
-    lxf.resize((size_t)kXfFloats * s->num_lights);
-    for (uint32_t l = 0; l < s->num_lights; l++) {
-        bool id;
-        if (!make_xf(s->lights[l].world_matrix, lxf.data() + (size_t)kXfFloats * l, id))
-            return lt_fail(ctx, LT_ERR_INVALID_ARG, "lights[" + std::to_string(l) + "].world_matrix is singular or not finite");
-        any_lxf |= !id && (s->lights[l].light_flags & 7u) == LT_LIGHT_AREA;
-    }
-    if (s->num_indices && s->num_prim_infos == 0) return lt_fail(ctx, LT_ERR_INVALID_ARG, "indices without meshes");
+    static const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    h.lxf.resize((size_t)kXfFloats * s->num_lights);
     for (uint32_t l = 0; l < s->num_lights; l++) {
         const lt_light& L = s->lights[l];
-        if ((L.light_flags & 7u) == LT_LIGHT_AREA) {
+        const bool area = (L.light_flags & 7u) == LT_LIGHT_AREA;
+        bool id;
+        // only area lights read their world matrix (commons.glsl:112-149)
+        if (!make_xf(area ? L.world_matrix : kIdentity, h.lxf.data() + (size_t)kXfFloats * l, id))
+            return bad(LT_ERR_INVALID_ARG, "lights[" + std::to_string(l) + "].world_matrix is singular or not finite");
+        h.any_lxf |= !id;
+        if (area) {
             if (L.prim_mesh_idx >= s->num_prim_infos)
-                return lt_fail(ctx, LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": mesh out of range");
+                return bad(LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": mesh out of range");
             uint64_t off, cnt;
             mesh_range(s, L.prim_mesh_idx, off, cnt);
             if (L.num_triangles == 0 || 3ull * L.num_triangles > cnt)
-                return lt_fail(ctx, LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": triangle count");
+                return bad(LT_ERR_INVALID_ARG, "light " + std::to_string(l) + ": triangle count");
         }
     }
     for (uint32_t m = 0; m < s->num_materials; m++) {
         int t = s->materials[m].texture_id;
         int r = s->roughness_texture_ids ? s->roughness_texture_ids[m] : -1;
         if (t >= (int)s->num_textures || r >= (int)s->num_textures)
-            return lt_fail(ctx, LT_ERR_INVALID_ARG, "material " + std::to_string(m) + ": texture out of range");
+            return bad(LT_ERR_INVALID_ARG, "material " + std::to_string(m) + ": texture out of range");
     }
     for (uint32_t t = 0; t < s->num_textures; t++)
         if (!s->textures[t].data || !s->textures[t].width || !s->textures[t].height)
-            return lt_fail(ctx, LT_ERR_INVALID_ARG, "texture " + std::to_string(t) + " is empty");
+            return bad(LT_ERR_INVALID_ARG, "texture " + std::to_string(t) + " is empty");
+    return LT_OK;
+}
+
+lt_status lt_scene_validate(const lt_scene_desc* s, char* err, size_t err_len) {
+    if (!s) return LT_ERR_INVALID_ARG;
+    SceneHost h;
+    std::string e;
+    const lt_status st = validate_scene(s, h, e);
+    if (err && err_len) {
+        strncpy(err, e.c_str(), err_len - 1);
+        err[err_len - 1] = 0;
+    }
+    return st;
+}
+
+lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* s) {
+    if (!ctx || !s) return LT_ERR_INVALID_ARG;
+    SceneHost h;
+    std::string e;
+    const lt_status vs = validate_scene(s, h, e);
+    if (vs != LT_OK) return lt_fail(ctx, vs, e);
+    const std::vector<uint32_t>& tri_start = h.tri_start;
+    const std::vector<float>&xf = h.xf, &lxf = h.lxf;
+    const bool any_xf = h.any_xf, any_lxf = h.any_lxf;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->generation++;

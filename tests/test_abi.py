@@ -147,3 +147,43 @@ def test_terrain_workload_size():
     # the boxes are instances of one 12-triangle cube: its geometry is stored once
     assert d.num_indices // 3 == 2 * 100 * 100 + 12 + 2
     assert d.num_prim_infos == 16 + 10 + 1 and bool(d.world_matrices) and bool(d.index_counts)
+
+
+def _validate(desc):
+    err = C.create_string_buffer(256)
+    st = la.lib().lt_scene_validate(C.byref(desc), err, 256)
+    return st, err.value.decode()
+
+
+@pytest.mark.parametrize("kind,kw", [("cornell", {}), ("caustics", dict(sphere_tris=2000)), ("diff_wall", {}),
+                                     ("terrain", dict(terrain_tris=2000, num_instances=20, num_point_lights=4))])
+def test_procedural_scenes_pass_upload_validation(kind, kw):
+    """lt_scene_validate runs the host-side checks of lt_scene_upload (index
+    ranges, references, instance transforms) without a device."""
+    sc = la.Scene.procedural(kind, seed=1, **kw)
+    assert _validate(sc.desc) == (0, "")
+
+
+def test_upload_validation_refusals():
+    from scenes import DescScene, quad
+    s = DescScene()
+    m = s.material(bsdf_type=1, bsdf_props=9, albedo=(0.5, 0.5, 0.5))
+    e = s.material(bsdf_type=1, bsdf_props=9, albedo=(1, 1, 1), emissive_factor=(2, 2, 2))
+    s.mesh(quad([-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]), m)
+    W = np.eye(4)
+    W[:3, 3] = (0, 0, 2)
+    s.instance(0, e, W)
+    s.analytic.append(dict(pos=(0, 3, 0), to=(0, 0, 0), L=(5, 5, 5), flags=4 | 16 | 32))  # a point light
+    d = s.build()
+    assert _validate(d) == (0, "")  # a point light's unset world matrix is not read
+    s._w[16 * 1 + 0] = 0.0  # instance 1: a singular matrix (x column zero)
+    st, msg = _validate(d)
+    assert st == 1 and "world_matrices[1]" in msg
+    s._w[16 * 1 + 0] = 1.0
+    s._c[1] = 9  # more indices than the geometry has
+    st, msg = _validate(d)
+    assert st == 1 and "prim_infos[1]" in msg
+    s._c[1] = 6
+    d.lights[0].num_triangles = 3  # the instance's light claims 3 triangles of 2
+    st, msg = _validate(d)
+    assert st == 1 and "triangle count" in msg
