@@ -449,6 +449,11 @@ def main() -> int:
     it.init()
     bst = it.create_accel()
     log(f"bvh8: {bst.num_nodes} nodes, depth {bst.max_depth}, build {bst.build_ms:.1f} ms on {arch} ({n_cu} CUs)")
+    # launch shapes from the autotuner's table for this device and workload, if it has them
+    from lumen_amd import autotune
+    tune_key = ("bdpt" if bdpt else f"vcm-{args.estimator}" if vcm else "sppm" if sppm else f"path-{args.strategy}")
+    tuned = autotune.apply(ctx, tune_key, kind)
+    log(f"tuning: {ctx.get_tuning()} ({'from the autotuner table' if tuned else 'defaults'})")
     depth = it.path_length
     sky = tuple(scene.integrator_cfg().sky_col)
     if bdpt and args.steps == 32 and "--steps" not in sys.argv:
@@ -603,6 +608,7 @@ def main() -> int:
             "rays_per_step": rays // args.steps,
             "closest_rays_per_step": c["closest"] // args.steps,
             "shadow_rays_per_step": (rays - c["closest"]) // args.steps,
+            "tuning": tuned,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -304,10 +304,16 @@ typedef struct lt_tuning {
                                       (default) or LT_WF_SORT_BSDF (the hit paths of a
                                       depth grouped by the BSDF type of their material
                                       before shading); 0 = unchanged                     */
-    uint32_t reserved[9];
+    uint32_t xcd_bands;            /* persistent Path kernel: LT_XCD_BANDS_OFF (default) or
+                                      LT_XCD_BANDS_ON (the pixels cut into 8 bands, one
+                                      per XCD's workgroups, each with its own counter);
+                                      0 = unchanged                                      */
+    uint32_t reserved[8];
 } lt_tuning;
 #define LT_WF_SORT_NONE 1u
 #define LT_WF_SORT_BSDF 2u
+#define LT_XCD_BANDS_OFF 1u
+#define LT_XCD_BANDS_ON 2u
 lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t);
 lt_status lt_get_tuning(lt_context* ctx, lt_tuning* t);
 

@@ -140,7 +140,7 @@ class Context:
         check(lib().lt_get_tuning(self._h, C.byref(t)), self._h, "lt_get_tuning")
         return {"path_blocks_per_cu": t.path_blocks_per_cu, "trace_blocks_per_cu": t.trace_blocks_per_cu,
                 "refill_min": t.refill_min, "bdpt_blocks_per_cu": t.bdpt_blocks_per_cu, "regen_min": t.regen_min,
-                "bdpt_chunk_tiles": t.bdpt_chunk_tiles, "wf_sort": t.wf_sort}
+                "bdpt_chunk_tiles": t.bdpt_chunk_tiles, "wf_sort": t.wf_sort, "xcd_bands": t.xcd_bands}
 
     def upload(self, scene: Scene) -> None:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")

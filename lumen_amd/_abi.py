@@ -6,11 +6,14 @@ if it is missing the import fails with an error telling how to build it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "liblt.so"
+# the built library (LT_LIBRARY names a variant build of the same sources,
+# as the autotuner's build-time sweep makes)
+LIB_PATH = Path(os.environ.get("LT_LIBRARY") or (Path(__file__).resolve().parent / "liblt.so"))
 
 # ---- status codes -----------------------------------------------------------
 LT_OK = 0
@@ -34,6 +37,7 @@ LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
 LT_WF_SORT_NONE, LT_WF_SORT_BSDF = 1, 2
 LT_COLLAPSE_SAH, LT_COLLAPSE_GREEDY = 1, 2
+LT_XCD_BANDS_OFF, LT_XCD_BANDS_ON = 1, 2
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
 
@@ -143,7 +147,8 @@ class lt_procedural_params(C.Structure):
 class lt_tuning(C.Structure):
     _fields_ = [("path_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("refill_min", C.c_uint32), ("bdpt_blocks_per_cu", C.c_uint32), ("regen_min", C.c_uint32),
-                ("bdpt_chunk_tiles", C.c_uint32), ("wf_sort", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+                ("bdpt_chunk_tiles", C.c_uint32), ("wf_sort", C.c_uint32), ("xcd_bands", C.c_uint32),
+                ("reserved", C.c_uint32 * 8)]
 
 
 class lt_scene_integrator_cfg(C.Structure):

@@ -64,9 +64,9 @@ def _resources(remarks: str) -> dict:
     return out
 
 
-def _compile(src: Path):
-    out = OBJ / (src.name + ".o")
-    cmd = [_hipcc(), *COMMON]
+def _compile(src: Path, obj: Path = OBJ, defines=()):
+    out = obj / (src.name + ".o")
+    cmd = [_hipcc(), *COMMON, *[f"-D{d}" for d in defines]]
     if src.suffix == ".hip":
         cmd += ["-x", "hip", *DEVICE, "-Rpass-analysis=kernel-resource-usage"]
     else:
@@ -79,26 +79,34 @@ def _compile(src: Path):
     return out, _resources(r.stderr) if src.suffix == ".hip" else {}
 
 
-def build(verbose: bool = False) -> Path:
-    """Compiles every source and links lumen_amd/liblt.so; returns its path."""
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, defines=(), lib_path: Path | None = None) -> Path:
+    """Compiles every source and links lumen_amd/liblt.so; returns its path.
+    defines (e.g. "LT_LDS_STACK=24") and lib_path build a variant library
+    elsewhere (the autotuner's build-time parameters); the default build
+    also links the headless CLI."""
+    variant = lib_path is not None
+    obj = OBJ / ("variant_" + "_".join(d.replace("=", "") for d in defines)) if variant else OBJ
+    obj.mkdir(parents=True, exist_ok=True)
     srcs = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp")])
     jobs = min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(_compile, srcs))
+        results = list(ex.map(lambda p: _compile(p, obj, defines), srcs))
     objs = [o for o, _ in results]
     # the device kernels' footprints (tests/test_kernel_footprint.py)
     res = {}
     for _, r in results:
         res.update(r)
-    (OBJ / "kernel_resources.json").write_text(json.dumps(res, indent=1, sort_keys=True))
-    tmp = LIB.with_suffix(".so.tmp")
+    (obj / "kernel_resources.json").write_text(json.dumps(res, indent=1, sort_keys=True))
+    lib = Path(lib_path) if variant else LIB
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
            "-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if variant:
+        return lib
     # headless C++ driver over the C ABI (host C++ only, links liblt.so)
     cli = PKG / "lt_render_cli"
     cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "host" / "lt_render_cli.cpp"),

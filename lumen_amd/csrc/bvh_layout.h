@@ -50,8 +50,17 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
 
-constexpr int kLdsStack = 16;     // stack entries per thread kept in LDS
-constexpr int kLeafQueue = 16;    // queued leaf entries per thread (LDS)
+// traversal stack entries per thread kept in LDS, and queued leaf entries per
+// thread (LDS); build-time parameters of the autotuner (lumen_amd/autotune.py)
+#ifndef LT_LDS_STACK
+#define LT_LDS_STACK 16
+#endif
+#ifndef LT_LEAF_QUEUE
+#define LT_LEAF_QUEUE 16
+#endif
+constexpr int kLdsStack = LT_LDS_STACK;
+constexpr int kLeafQueue = LT_LEAF_QUEUE;
+static_assert(kLdsStack >= 4 && kLdsStack <= 32 && kLeafQueue >= 8 && kLeafQueue <= 32, "LDS stack / queue sizes");
 constexpr int kLdsWords = kLdsStack + kLeafQueue;  // LDS words per thread of a traversal
 constexpr int kStackTotal = 192;  // LDS + global spill entries per thread
 constexpr int kMaxLeafTris = 7;   // count field is 3 bits

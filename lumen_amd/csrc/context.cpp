@@ -430,6 +430,7 @@ lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
     if (t->refill_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "refill_min must be in 1..64");
     if (t->regen_min > 64) return lt_fail(ctx, LT_ERR_INVALID_ARG, "regen_min must be in 1..64");
     if (t->wf_sort > LT_WF_SORT_BSDF) return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown wf_sort");
+    if (t->xcd_bands > LT_XCD_BANDS_ON) return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown xcd_bands");
     if (t->path_blocks_per_cu) ctx->tuning.path_blocks_per_cu = t->path_blocks_per_cu;
     if (t->bdpt_blocks_per_cu) ctx->tuning.bdpt_blocks_per_cu = t->bdpt_blocks_per_cu;
     if (t->trace_blocks_per_cu) ctx->tuning.trace_blocks_per_cu = t->trace_blocks_per_cu;
@@ -437,6 +438,7 @@ lt_status lt_set_tuning(lt_context* ctx, const lt_tuning* t) {
     if (t->regen_min) ctx->tuning.regen_min = t->regen_min;
     if (t->bdpt_chunk_tiles) ctx->tuning.bdpt_chunk_tiles = t->bdpt_chunk_tiles;
     if (t->wf_sort) ctx->tuning.wf_sort = t->wf_sort;
+    if (t->xcd_bands) ctx->tuning.xcd_bands = t->xcd_bands;
     ctx->generation++;
     return LT_OK;
 }

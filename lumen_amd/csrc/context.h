@@ -110,7 +110,7 @@ struct lt_context {
     // by every context on this device; not owned)
     uint32_t* noalloc_scratch = nullptr;
     float last_ms = 0.0f;
-    lt_tuning tuning{10u, 8u, 32u, 0u, 4u, 0u, LT_WF_SORT_NONE, {}};
+    lt_tuning tuning{10u, 8u, 32u, 0u, 4u, 0u, LT_WF_SORT_NONE, LT_XCD_BANDS_OFF, {}};
     // bumped whenever device state that launches capture by pointer or value
     // changes (scene upload, BVH build, spill reallocation, launch shapes):
     // a captured frame graph older than this is stale

@@ -20,6 +20,7 @@ struct PathLaunch {
     const uint32_t* frame_dev;
     uint32_t regen_min;  // persistent kernel: idle lanes that trigger a refill
     uint32_t nee_pick;   // LT_NEE_REFERENCE or LT_NEE_CORRECTED
+    uint32_t xcd_bands;  // persistent kernel: pixels drawn per XCD from 8 bands
 };
 
 // Probability of the NEE light pick that sample_light_Li made (light `li`):

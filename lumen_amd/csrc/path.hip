@@ -187,6 +187,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     const uint32_t frame_word = frame_num ^ L.seed;
     Ctr c{0, 0, 0, 0};
     PathState p;
+    // XCD bands: workgroups are dealt to the 8 XCDs round-robin (blockIdx mod
+    // 8); with bands on, the pixels are cut into 8 contiguous bands of tiles
+    // with a counter each, and a wave takes pixels from its XCD's band, so the
+    // primary rays of one L2 share a compact image region; a wave whose band
+    // is used up helps the next band.  Off: one counter over every pixel.
+    const uint32_t n_bands = (L.xcd_bands && (gridDim.x & 7u) == 0u) ? 8u : 1u;
+    const uint32_t band_len = (n_items + n_bands - 1u) / n_bands;
+    uint32_t band = n_bands == 8u ? (blockIdx.x & 7u) : 0u, bands_done = 0;  // wave-uniform
     bool has = false, exhausted = false;  // exhausted is wave-uniform
     for (;;) {
         if (!exhausted) {
@@ -195,12 +203,17 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
             if (n_idle >= L.regen_min || n_idle == 64u) {
                 const int leader = __ffsll((long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(work, n_idle);
+                if ((int)lane == leader) base = atomicAdd(work + band, n_idle);
                 base = __shfl(base, leader);
-                if (base + n_idle >= n_items) exhausted = true;
+                const uint32_t band_lo = band * band_len, band_hi = min(n_items, band_lo + band_len);
+                const uint32_t first = band_lo + base;
+                if (first + n_idle >= band_hi) {  // this band is used up: the next one, until all are
+                    band = band + 1u == n_bands ? 0u : band + 1u;
+                    if (++bands_done == n_bands) exhausted = true;
+                }
                 if (!has) {
-                    const uint32_t t = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                    if (t < n_items) {
+                    const uint32_t t = first + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (t < band_hi) {
                         const uint32_t tile = t >> 6, w = t & 63u;
                         const uint32_t x = (tile % tiles_x) * 8 + (w & 7u), y = (tile / tiles_x) * 8 + (w >> 3);
                         if (x < L.width && y < L.height) {
@@ -280,7 +293,8 @@ lt_status launch_path_persistent(lt_context* ctx, const PathLaunch& L0, float4* 
         L.frame_start = L0.frame_start + f;
         L.spp = 1;
         L.regen_min = ctx->tuning.regen_min ? ctx->tuning.regen_min : 4u;
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 4u);
+        L.xcd_bands = ctx->tuning.xcd_bands == LT_XCD_BANDS_ON ? 1u : 0u;
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, 8u);
         if (count)
             hipLaunchKernelGGL((k_path_regen<true>), dim3(blocks), dim3(kTraceBlock), 0, ctx->stream, sv, L, accum,
                                ctx->counters, ctx->spill, cnt);

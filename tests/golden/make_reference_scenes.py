@@ -1,0 +1,27 @@
+"""Copies the reference's own scene files used by the tests into
+tests/golden/scenes (data fixtures: the Cornell box of scenes/cornell_box with
+its Path / BDPT / VCM / SPPM integrator blocks, and scenes/caustics; the OBJ
+meshes gzip-compressed).  Run in a checkout that has the reference at
+/root/reference."""
+import gzip
+import shutil
+from pathlib import Path
+
+SRC = Path("/root/reference/scenes")
+DST = Path(__file__).resolve().parent / "scenes"
+
+FILES = ["cornell_box/cornell_box_path.json", "cornell_box/cornell_box_bdpt.json", "cornell_box/cornell_box_vcm.json",
+         "cornell_box/cornell_box_sppm.json", "caustics.json"]
+MESHES = ["cornell_box/cornell_box.obj", "caustics.obj"]
+
+
+def main():
+    DST.mkdir(exist_ok=True)
+    for f in FILES:
+        shutil.copyfile(SRC / f, DST / Path(f).name)
+    for f in MESHES:
+        (DST / (Path(f).name + ".gz")).write_bytes(gzip.compress((SRC / f).read_bytes(), 9, mtime=0))
+
+
+if __name__ == "__main__":
+    main()
