@@ -626,6 +626,14 @@ lt_status lt_exr_save(const char* path, const float* rgba, uint32_t width, uint3
  * (src/RayTracer/RayTracer.cpp:199-242). */
 lt_status lt_exr_load(const char* path, float* rgba, size_t capacity, uint32_t* width, uint32_t* height, char* err,
                       size_t err_len);
+/* Decodes a baseline (sequential Huffman, 8-bit) JPEG file into 8-bit RGBA,
+ * alpha 255, rows top to bottom: the texture decode of the scene loader,
+ * stbi_load(path, &x, &y, &n, 4) in src/RayTracer/LumenScene.cpp:205-212.
+ * With rgba == NULL only the size is returned; otherwise capacity (bytes)
+ * must be at least 4 * width * height.  Progressive, arithmetic-coded and
+ * 12-bit files return LT_ERR_UNSUPPORTED. */
+lt_status lt_image_load_rgba8(const char* path, uint8_t* rgba, size_t capacity, uint32_t* width, uint32_t* height,
+                              char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -506,3 +506,17 @@ def load_exr(path) -> np.ndarray:
     st = lib().lt_exr_load(str(path).encode(), img.ctypes.data, img.size, C.byref(w), C.byref(h), err, 512)
     check(st, what=f"lt_exr_load({path}): {err.value.decode()}")
     return img
+
+
+def load_image_rgba8(path) -> np.ndarray:
+    """Decodes a baseline JPEG texture as an (H, W, 4) uint8 image, row 0 at
+    the top, alpha 255 (the scene loader's texture decode; the reference's
+    stbi_load(..., 4), src/RayTracer/LumenScene.cpp:205-212)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(512)
+    st = lib().lt_image_load_rgba8(str(path).encode(), None, 0, C.byref(w), C.byref(h), err, 512)
+    check(st, what=f"lt_image_load_rgba8({path}): {err.value.decode()}")
+    img = np.empty((h.value, w.value, 4), np.uint8)
+    st = lib().lt_image_load_rgba8(str(path).encode(), img.ctypes.data, img.size, C.byref(w), C.byref(h), err, 512)
+    check(st, what=f"lt_image_load_rgba8({path}): {err.value.decode()}")
+    return img

@@ -212,6 +212,8 @@ _PROTOS = {
     "lt_exr_save": (C.c_int, [C.c_char_p, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_char_p, C.c_size_t]),
     "lt_exr_load": (C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_char_p,
                               C.c_size_t]),
+    "lt_image_load_rgba8": (C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.c_char_p, C.c_size_t]),
 }
 EXPORTED = tuple(_PROTOS)
 

@@ -396,8 +396,7 @@ void finalize_scene(lt_host_scene& s, const std::vector<AnalyticLight>& analytic
     }
     s.total_light_area = area;
     if (s.rough_tex.size() < s.materials.size()) s.rough_tex.resize(s.materials.size(), -1);
-    s.textures.clear();
-    for (auto& td : s.tex_data) (void)td;
+    s.textures.clear();  // views are built by texture_views once tex_data is complete
 }
 
 // ------------------------------------------------------------ helpers ----
@@ -911,19 +910,27 @@ lt_status make_procedural(lt_procedural_kind kind, const lt_procedural_params* p
         case LT_SCENE_CAUSTICS: s = make_caustics(p); break;
         default: return LT_ERR_INVALID_ARG;
     }
-    // texture views
-    s->textures.clear();
+    // the procedural textures are square
+    s->tex_dims.clear();
     for (auto& td : s->tex_data) {
-        lt_texture t;
-        uint32_t n = (uint32_t)(td.size() / 4);
-        uint32_t r = (uint32_t)lroundf(sqrtf((float)n));
-        t.width = r;
-        t.height = r;
-        t.data = td.data();
-        s->textures.push_back(t);
+        const uint32_t r = (uint32_t)lroundf(sqrtf((float)(td.size() / 4)));
+        s->tex_dims.push_back(r);
+        s->tex_dims.push_back(r);
     }
+    texture_views(*s);
     *out = s;
     return LT_OK;
+}
+
+void texture_views(lt_host_scene& s) {
+    s.textures.clear();
+    for (size_t i = 0; i < s.tex_data.size(); i++) {
+        lt_texture t;
+        t.width = s.tex_dims[2 * i];
+        t.height = s.tex_dims[2 * i + 1];
+        t.data = s.tex_data[i].data();
+        s.textures.push_back(t);
+    }
 }
 
 }  // namespace lt

@@ -16,6 +16,7 @@ struct lt_host_scene {
     std::vector<lt_light> lights;
     std::vector<float> spot_cones;  // 2 per light
     std::vector<std::vector<float>> tex_data;
+    std::vector<uint32_t> tex_dims;  // width, height per texture
     std::vector<lt_texture> textures;
     std::vector<int32_t> rough_tex;  // per material
     // per prim mesh, empty unless some mesh is an instance: world matrix (16
@@ -67,6 +68,12 @@ void finalize_scene(lt_host_scene& s, const std::vector<AnalyticLight>& analytic
 void make_camera(float fov_deg, const float pos[3], const float dir[3], float aspect, lt_camera* out);
 
 lt_status load_json_scene(const char* path, lt_host_scene** out, std::string& err);
+// (re)builds s.textures, the lt_texture views of s.tex_data
+void texture_views(lt_host_scene& s);
+// Decodes a baseline JPEG byte stream into 8-bit RGBA rows, top to bottom
+// (image_jpeg.cpp).
+lt_status decode_jpeg(const uint8_t* data, size_t n, std::vector<uint8_t>& rgba, uint32_t& w, uint32_t& h,
+                      std::string& err);
 lt_status make_procedural(lt_procedural_kind kind, const lt_procedural_params* p, lt_host_scene** out);
 
 }  // namespace lt

@@ -7,15 +7,19 @@
 // `o` and `g` starting a new shape (tiny_obj_loader.h:2770-2840), quads split
 // along their shorter diagonal (:1427-1531).  Polygons with more than four
 // vertices are fanned from their first vertex (tinyobjloader ear-clips them;
-// none of the reference's scene files has such faces).  Image textures
-// ("texture" keys, decoded by stb_image in the reference) are not supported:
-// float textures are passed through lt_scene_desc instead.
+// none of the reference's scene files has such faces).  A material's
+// "texture" (LumenScene.cpp:357-360, a path relative to the JSON file) is
+// decoded as a baseline JPEG into 8-bit RGBA (image_jpeg.cpp) and stored as
+// linear floats through the sRGB transfer function, the conversion that
+// sampling a VK_FORMAT_R8G8B8A8_SRGB image applies before filtering
+// (LumenScene.cpp:205-212); each reference adds its own texture, as there.
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -322,12 +326,49 @@ bool load_obj(const std::string& path, std::vector<ObjShape>& shapes, std::strin
 
 // ------------------------------------------------------------- materials ----
 // LumenScene.cpp:350-475
-bool make_material(const JVal& b, lt_material& m, std::string& err) {
+// sRGB transfer function (IEC 61966-2-1), 8-bit code -> linear
+float srgb_to_linear(uint32_t c) {
+    const double v = c / 255.0;
+    return (float)(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+}
+
+bool load_texture(const std::string& path, lt_host_scene& s, int32_t& id, std::string& err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open texture " + path;
+        return false;
+    }
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<uint8_t> rgba;
+    uint32_t w = 0, h = 0;
+    std::string derr;
+    if (decode_jpeg(bytes.data(), bytes.size(), rgba, w, h, derr) != LT_OK) {
+        err = path + ": " + derr;
+        return false;
+    }
+    float lut[256];
+    for (uint32_t c = 0; c < 256; c++) lut[c] = srgb_to_linear(c);
+    std::vector<float> texels((size_t)w * h * 4);
+    for (size_t k = 0; k < (size_t)w * h; k++) {
+        for (int c = 0; c < 3; c++) texels[4 * k + c] = lut[rgba[4 * k + c]];
+        texels[4 * k + 3] = rgba[4 * k + 3] / 255.0f;  // alpha is stored linear
+    }
+    s.tex_data.push_back(std::move(texels));
+    s.tex_dims.push_back(w);
+    s.tex_dims.push_back(h);
+    id = (int32_t)s.tex_data.size() - 1;
+    return true;
+}
+
+bool make_material(const JVal& b, lt_material& m, const std::string& dir, lt_host_scene& s, std::string& err) {
     m = lt_material{};
     m.texture_id = -1;
-    if (b.get("texture")) {
-        err = "image textures (\"texture\") are not supported by this loader";
-        return false;
+    if (const JVal* tex = b.get("texture")) {
+        if (tex->kind != JVal::STR) {
+            err = "\"texture\" is not a string";
+            return false;
+        }
+        if (!load_texture(dir + tex->str, s, m.texture_id, err)) return false;
     }
     if (!vec3(b.get("albedo"), m.albedo)) m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
     vec3(b.get("emissive_factor"), m.emissive_factor);
@@ -468,7 +509,7 @@ lt_status load_json_scene(const char* path, lt_host_scene** out, std::string& er
     if (bsdfs && bsdfs->kind == JVal::ARR) {
         for (size_t i = 0; i < bsdfs->arr.size(); i++) {
             lt_material m;
-            if (!make_material(bsdfs->arr[i], m, err)) {
+            if (!make_material(bsdfs->arr[i], m, dir, *s, err)) {
                 err = "bsdf " + std::to_string(i) + ": " + err;
                 return LT_ERR_PARSE;
             }
@@ -514,6 +555,7 @@ lt_status load_json_scene(const char* path, lt_host_scene** out, std::string& er
         }
     }
     finalize_scene(*s, al);
+    texture_views(*s);
     *out = s.release();
     return LT_OK;
 }

@@ -16,11 +16,36 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
 
 
+REF_TREE = Path("/root/reference")
+REF_STBI = HERE / "_ref" / "libref_stbi.so"
+
+
 def build() -> Path:
     r = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
+    if (REF_TREE / "libs" / "stb_image" / "stb_image.h").exists():
+        # the reference's own decoder, compiled from its tree (oracle/_ref)
+        r = subprocess.run(["make", "-C", str(HERE), "ref", f"REF={REF_TREE}"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"oracle/_ref build failed\n{r.stdout}\n{r.stderr}")
     return LIB_PATH
+
+
+def ref_stbi_load(path: str):
+    """RGBA8 image decoded by the reference's stb_image (oracle/_ref), or
+    None where that library is not built."""
+    if not REF_STBI.exists():
+        return None
+    L = C.CDLL(str(REF_STBI))
+    L.ref_stbi_load_rgba8.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    w, h = C.c_int(), C.c_int()
+    if L.ref_stbi_load_rgba8(path.encode(), None, 0, C.byref(w), C.byref(h)) != 0:
+        raise RuntimeError(f"stb_image cannot decode {path}")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    if L.ref_stbi_load_rgba8(path.encode(), out.ctypes.data, out.size, C.byref(w), C.byref(h)) != 0:
+        raise RuntimeError(f"stb_image cannot decode {path}")
+    return out
 
 
 class PathParams(C.Structure):

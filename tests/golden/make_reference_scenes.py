@@ -1,7 +1,8 @@
 """Copies the reference's own scene files used by the tests into
 tests/golden/scenes (data fixtures: the Cornell box of scenes/cornell_box with
-its Path / BDPT / VCM / SPPM integrator blocks, and scenes/caustics; the OBJ
-meshes gzip-compressed).  Run in a checkout that has the reference at
+its Path / BDPT / VCM / SPPM integrator blocks and the directional-light
+variant with its wood texture, and scenes/caustics; the OBJ meshes
+gzip-compressed).  Run in a checkout that has the reference at
 /root/reference."""
 import gzip
 import shutil
@@ -11,7 +12,8 @@ SRC = Path("/root/reference/scenes")
 DST = Path(__file__).resolve().parent / "scenes"
 
 FILES = ["cornell_box/cornell_box_path.json", "cornell_box/cornell_box_bdpt.json", "cornell_box/cornell_box_vcm.json",
-         "cornell_box/cornell_box_sppm.json", "caustics.json"]
+         "cornell_box/cornell_box_sppm.json", "cornell_box/cornell_box_dir.json", "cornell_box/wood1.jpg",
+         "caustics.json"]
 MESHES = ["cornell_box/cornell_box.obj", "caustics.obj"]
 
 

@@ -1,0 +1,76 @@
+"""Ground-truth validation: independent GPU estimates converge to the
+oracle's expectation (the time-to-RMSE ground truths are GPU renders, so they
+are checked here against the CPU oracle in distribution, not sample by sample).
+
+Both sides render independent 1-spp images (different seeds, so no replayed
+random stream is shared); per 8x8 block the difference of the two sample
+means is divided by its standard error from the two samples' variances.
+Under the hypothesis of equal expectations these z values are about
+standard normal: at most one of the 64 blocks may exceed 4 (a normal tail
+of 6e-5 per block; the sample variances of heavy-tailed estimators make the
+tails of z somewhat wider), their mean square must stay near 1 and their
+sum within 4 standard deviations."""
+import numpy as np
+import pytest
+
+import lumen_amd as la
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _z_blocks(a, b, bs=8):
+    """a, b: [n, H, W] samples of one channel sum; block means and their z."""
+    na, nb = a.shape[0], b.shape[0]
+    H, W = a.shape[1], a.shape[2]
+    ab = a.reshape(na, H // bs, bs, W // bs, bs).mean(axis=(2, 4))
+    bb = b.reshape(nb, H // bs, bs, W // bs, bs).mean(axis=(2, 4))
+    se = np.sqrt(ab.var(axis=0, ddof=1) / na + bb.var(axis=0, ddof=1) / nb)
+    ok = se > 0
+    return (ab.mean(0)[ok] - bb.mean(0)[ok]) / se[ok]
+
+
+def _accept(z) -> bool:
+    """No more than one block beyond 4 sigma, mean square near 1, and the
+    blocks' summed z within 4 sigma (a global bias of ~3 % on the procedural
+    Cornell box fails this one)."""
+    return (np.abs(z) > 4.0).sum() <= 1 and np.mean(z ** 2) <= 1.6 and abs(z.sum()) / np.sqrt(len(z)) <= 4.0
+
+
+@pytest.mark.parametrize("integrator,kind", [("path", "cornell"), ("bdpt", "cornell"), ("vcm", "caustics")])
+def test_gpu_estimate_matches_oracle_in_distribution(gpu_ctx, integrator, kind):
+    W, H = 64, 64
+    sc = la.Scene.procedural(kind, seed=1)
+    n_gpu, n_cpu = 256, 64
+    print(f"[gpu] ground truth {integrator} {kind}: {n_gpu} GPU vs {n_cpu} oracle images", flush=True)
+    kw = dict(path_length=5)
+    if integrator == "vcm":
+        kw.update(radius_factor=2.0, enable_vm=True, estimator="corrected")
+    it = la.create_integrator(integrator, sc, gpu_ctx, W, H, **kw)
+    it.init()
+    it.create_accel()
+    g = np.zeros((n_gpu, H, W), np.float64)
+    for k in range(n_gpu):
+        it.seed = 1000 + k
+        it.reset()
+        it.render(spp=1)
+        g[k] = it.output_tex[..., :3].sum(-1)
+    cam, depth, sky = it.camera, it.path_length, tuple(getattr(it, "sky_col", (0, 0, 0)))
+    it.destroy()
+    acc = oracle.Accel(sc.desc)
+    c = np.zeros((n_cpu, H, W), np.float64)
+    for k in range(n_cpu):
+        s = 50000 + k
+        if integrator == "path":
+            o, _, _, _ = acc.render_path(cam, W, H, depth, sky_col=sky, frames=1, seed=s)
+        elif integrator == "bdpt":
+            o, _, _, _ = acc.render_bdpt(cam, W, H, depth, frames=1, seed=s)
+        else:
+            o, _, _, _ = acc.render_vcm(cam, W, H, depth, frames=1, seed=s, radius_factor=2.0, use_vm=True,
+                                        corrected=True, sky_col=sky)
+        c[k] = o[..., :3].sum(-1)
+    z = _z_blocks(g, c)
+    print(f"[gpu] {integrator} {kind}: {len(z)} blocks, max |z| {np.abs(z).max():.2f}, "
+          f"mean z^2 {np.mean(z ** 2):.3f}, summed z {z.sum() / np.sqrt(len(z)):.2f}", flush=True)
+    assert len(z) >= 48
+    assert _accept(z)
