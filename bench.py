@@ -330,6 +330,32 @@ def bench_adjoint(args) -> int:
     paths = V * W * H * spp
     ms = wall / steps * 1e3
     log(f"timed {steps} iterations in {wall:.3f}s: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, loss {loss:.4g}")
+    # algorithmic bytes of one iteration from the counting variants of the
+    # same kernels (untimed): BVH8 nodes x 128 B + triangles x 48 B, plus the
+    # radiance cache (12 B per sample, written forward and read backward) and
+    # the images / their adjoints (12 B per pixel and variant); texel reads and
+    # the gradient atomics are not counted
+    adj.set_counting(True)
+    img_c, cache_c = adj.forward(albs, roughs, seed=warmup + steps)
+    cf = adj.counts()
+    adj.backward(albs, roughs, 2.0 * (img_c - target), seed=warmup + steps, cache=cache_c)
+    cb = adj.counts()
+    adj.set_counting(False)
+    cache_b = 12 * spp * V * W * H
+    img_b = 12 * V * W * H
+    fwd_bytes = cf["nodes"] * 128 + cf["tris"] * 48 + cache_b + img_b
+    bwd_bytes = cb["nodes"] * 128 + cb["tris"] * 48 + cache_b + img_b
+
+    def line(kernel, nbytes, kms, counts):
+        ach = nbytes / (kms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "kernel_ms_avg": round(kms, 3),
+                "algorithmic_bytes_per_launch": int(nbytes), "rays_per_launch": counts["closest"] + counts["shadow"]}
+
+    roof = line("k_adjoint<true> (path replay with texel-gradient atomics)", bwd_bytes, bwd_ms, cb)
+    roof["other_kernels"] = {"k_adjoint<false>": line("k_adjoint<false> (forward, writes the radiance cache)",
+                                                      fwd_bytes, fwd_ms, cf)}
+    log(f"adjoint roofline: {roof}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_adjoint(scene, adj.camera, W, H, depth, alb0, rough0)
@@ -353,7 +379,7 @@ def bench_adjoint(args) -> int:
             "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
             "mpaths_per_s_forward_plus_replay": round(2 * paths / (ms * 1e-3) / 1e6, 1),
         },
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -217,8 +217,8 @@ lt_status lt_device_info(lt_context* ctx, int* num_cus, char* arch, size_t arch_
 /* Replaces LumenScene::load_scene's buffer uploads (LumenScene.cpp:146-216). */
 lt_status lt_scene_upload(lt_context* ctx, const lt_scene_desc* scene);
 /* The host-side checks of lt_scene_upload without a device (index ranges,
- * references, transforms); err receives the reason of a refusal.  No
- * reference counterpart. */
+ * references, transforms, finite vertex positions); err receives the reason
+ * of a refusal.  No reference counterpart. */
 lt_status lt_scene_validate(const lt_scene_desc* scene, char* err, size_t err_len);
 
 /* Build options for lt_accel_build. */
@@ -545,6 +545,13 @@ lt_status lt_adjoint_forward(lt_adjoint* adj, const lt_camera* camera, uint32_t 
 lt_status lt_adjoint_backward(lt_adjoint* adj, const lt_camera* camera, uint32_t seed, const float* albedo,
                               const float* roughness, const float* d_images, const float* sample_cache,
                               float* grad_albedo, float* grad_roughness, void* stream);
+/* Counting mode (no counterpart in the reference; for measurement): while on,
+ * forward and backward calls run the counting variants of their kernels and
+ * record the work of the call; lt_adjoint_counts waits for the device and
+ * returns that of the last call: closest-hit rays, occlusion rays, BVH8 nodes
+ * and triangles visited by all queries, and those of the occlusion queries. */
+lt_status lt_adjoint_set_counting(lt_adjoint* adj, int32_t on);
+lt_status lt_adjoint_counts(lt_adjoint* adj, uint64_t counts[6]);
 
 /* ---- metrics ------------------------------------------------------------------- */
 

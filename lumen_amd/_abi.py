@@ -201,6 +201,8 @@ _PROTOS = {
     "lt_adjoint_cache_floats": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "lt_adjoint_forward": (C.c_int, [P, C.POINTER(lt_camera), C.c_uint32, P, P, P, P, P]),
     "lt_adjoint_backward": (C.c_int, [P, C.POINTER(lt_camera), C.c_uint32, P, P, P, P, P, P, P]),
+    "lt_adjoint_set_counting": (C.c_int, [P, C.c_int32]),
+    "lt_adjoint_counts": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "lt_image_sq_error": (C.c_int, [P, P, P, C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),
     "lt_scene_procedural": (C.c_int, [C.c_int, C.POINTER(lt_procedural_params), C.POINTER(P)]),
     "lt_scene_load_json": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),

@@ -99,6 +99,16 @@ class Adjoint:
               self.ctx.handle, "lt_adjoint_backward")
         return ga, gr
 
+    def set_counting(self, on: bool) -> None:
+        """Counting variants of the kernels for later calls (measurement)."""
+        check(lib().lt_adjoint_set_counting(self._h, int(on)), self.ctx.handle, "lt_adjoint_set_counting")
+
+    def counts(self) -> dict:
+        """Work of the last forward / backward call made while counting."""
+        c = (C.c_uint64 * 6)()
+        check(lib().lt_adjoint_counts(self._h, c), self.ctx.handle, "lt_adjoint_counts")
+        return dict(zip(("closest", "shadow", "nodes", "tris", "shadow_nodes", "shadow_tris"), c[:]))
+
     def close(self):
         if getattr(self, "_h", None):
             lib().lt_adjoint_destroy(self._h)

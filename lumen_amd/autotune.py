@@ -41,8 +41,8 @@ SPACE = {
     "path-wavefront": {"path_blocks_per_cu": [6, 8, 10, 12, 16], "refill_min": [8, 16, 32, 48, 64],
                        "wf_sort": [1, 2]},
     "bdpt": {"bdpt_blocks_per_cu": [2, 3, 4, 6, 8], "regen_min": [1, 2, 4, 8, 16]},
-    "vcm-corrected": {"bdpt_blocks_per_cu": [2, 3, 4, 6]},
-    "sppm": {"bdpt_blocks_per_cu": [2, 3, 4, 6]},
+    "vcm-corrected": {"bdpt_blocks_per_cu": [2, 3, 4, 6, 8]},
+    "sppm": {"bdpt_blocks_per_cu": [2, 3, 4, 6, 8]},
 }
 
 # build-time sizes (preprocessor definitions of bvh_layout.h / dev_trace.h)
@@ -61,11 +61,12 @@ def _time_frames(integrator, frames: int) -> float:
 
 
 def tune_field(integrator, field: str, candidates, frames: int = 4) -> dict:
-    """Times `frames` 1-spp frames per value of one lt_tuning field; returns
-    {"best": value, "ms": {value: avg ms}} and sets the best."""
+    """Times `frames` 1-spp frames per value of one lt_tuning field (the
+    candidates and the current value); returns {"best": value, "ms": {value:
+    avg ms}} and sets the best."""
     ctx = integrator.ctx
     results = {}
-    for c in candidates:
+    for c in sorted(set(candidates) | {ctx.get_tuning()[field]}):
         ctx.set_tuning(**{field: c})
         results[c] = _time_frames(integrator, frames)
     best = min(results, key=results.get)

@@ -41,5 +41,6 @@ struct AdjBuffers {
     float* g_rough;
 };
 
+// counts: NULL, or kCountSlots u64 that the counting kernel variants fill
 lt_status launch_adjoint(lt_context* ctx, const AdjLaunch& L, const AdjBuffers& B, bool backward, uint32_t* work,
-                         uint32_t* spill, uint32_t blocks, hipStream_t stream);
+                         uint32_t* spill, uint32_t blocks, unsigned long long* counts, hipStream_t stream);

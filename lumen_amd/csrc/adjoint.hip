@@ -34,6 +34,7 @@
 // traversals run beside a long per-lane replay state, and the leaf queue's
 // deferred culling cost them more node steps than it saved in idle lanes.
 #define LT_NODE_PEND_MAX 0
+#include "dev_count.h"
 #include "dev_trace.h"
 
 using namespace ltd;
@@ -253,26 +254,29 @@ LT_DEV f3 safe_inv(f3 f) {
     return {f.x != 0.0f ? 1.0f / f.x : 0.0f, f.y != 0.0f ? 1.0f / f.y : 0.0f, f.z != 0.0f ? 1.0f / f.z : 0.0f};
 }
 
-LT_DEV void closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl) {
+template <bool COUNT>
+LT_DEV void closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl, Ctr& c) {
     HitRec h;
-    uint32_t nn = 0, nt = 0;
-    if (trace_bvh<false, false>(sv, o, d, ADJ_TMIN, ADJ_TMAX, h, stk, nn, nt))
+    if (COUNT) c.closest++;
+    if (trace_bvh<false, COUNT>(sv, o, d, ADJ_TMIN, ADJ_TMAX, h, stk, c.nodes, c.tris))
         fill_payload(sv, h.inst, h.prim, h.u, h.v, pl);
     else
         pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
 }
 
-LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk) {
+template <bool COUNT>
+LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk, Ctr& c) {
     HitRec h;
-    uint32_t nn = 0, nt = 0;
-    return trace_bvh<true, false>(sv, o, d, 0.0f, tmax, h, stk, nn, nt);
+    if (COUNT) c.shadow++;
+    return trace_bvh<true, COUNT>(sv, o, d, 0.0f, tmax, h, stk, c.snodes, c.stris);
 }
 
 // One sample of pixel (x, y).  REPLAY: the same path again with its radiance
 // Lt and adjoint radiance delta, scattering texel gradients into ga / gr.
-template <bool REPLAY>
+template <bool REPLAY, bool COUNT>
 LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, const float* rgh, uint32_t x,
-                     uint32_t y, uint32_t word, f3 Lt, f3 delta, float* ga, float* gr, const TraceStack& stk) {
+                     uint32_t y, uint32_t word, f3 Lt, f3 delta, float* ga, float* gr, const TraceStack& stk,
+                     Ctr& cnt) {
     RNG seed = {x, y, word, 0u};
     const float r0 = rnd(seed);
     const float r1 = rnd(seed);
@@ -292,7 +296,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
     pl.pos = pl.n_s = pl.n_g = mk3(0.0f);
     const float pick = 1.0f / (float)sv.total_light_triangle_cnt;
     for (int depth = 0;; depth++) {
-        closest(sv, origin, direction, stk, pl);
+        closest<COUNT>(sv, origin, direction, stk, pl, cnt);
         if (!pl.valid) {
             col += thr * ld3(L.sky_col);
             break;
@@ -335,7 +339,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                     const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bpdf / pw);
                     const f3 ka = mis * fabsf(cos_x) * Le / pw;
                     const f3 contrib = mis * fl.f * fabsf(cos_x) * Le / pw;
-                    if ((nz(contrib) || nz(fl.dfa) || nz(fl.dfr)) && !occluded(sv, pp, wi, wi_len - EPS, stk)) {
+                    if ((nz(contrib) || nz(fl.dfa) || nz(fl.dfr)) && !occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt)) {
                         res += contrib;
                         const f3 w = delta * thr / pick * ka;
                         qa += w * fl.dfa;
@@ -348,7 +352,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                 if (pw > 0.0f) {
                     const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bpdf / pw);
                     const f3 contrib = mis * f * fabsf(cos_x) * Le / pw;
-                    if (nz(contrib) && !occluded(sv, pp, wi, wi_len - EPS, stk)) res += contrib;
+                    if (nz(contrib) && !occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt)) res += contrib;
                 }
             }
             if ((rec.flags & 7u) == LT_LIGHT_AREA) {
@@ -359,7 +363,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                                      : DF{sample_bsdf(n_s, wo, dm.m, 1, side, wb, bp, cx, s0, s1, s2), mk3(0.0f),
                                           mk3(0.0f)};
                 if (bp != 0.0f) {
-                    closest(sv, pp, wb, stk, pl);
+                    closest<COUNT>(sv, pp, wb, stk, pl, cnt);
                     if (pl.triangle_idx == rec.triangle_idx && pl.instance_idx == rec.instance_idx) {
                         const float wl = length(pl.pos - hp.pos);
                         const float gg = fabsf(dot(pl.n_s, -wb)) / (wl * wl);
@@ -404,9 +408,13 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
 #ifndef LT_ADJ_WAVES
 #define LT_ADJ_WAVES 3
 #endif
-template <bool BACKWARD>
+// COUNT: the counting variant (rays traced, BVH8 nodes and triangles visited,
+// summed into counts)
+template <bool BACKWARD, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_ADJ_WAVES))) void k_adjoint(SceneView sv, AdjLaunch L, AdjBuffers B,
-                                                         uint32_t* __restrict__ work, uint32_t* __restrict__ spill) {
+                                                         uint32_t* __restrict__ work, uint32_t* __restrict__ spill,
+                                                         unsigned long long* __restrict__ counts) {
+    Ctr cnt{0u, 0u, 0u, 0u, 0u, 0u};
     __shared__ uint32_t s_stack[ltb::kLdsWords * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const TraceStack stk{s_stack + threadIdx.x, spill + (size_t)gtid * (ltb::kStackTotal - ltb::kLdsStack)};
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(work, 1u);
         t = __shfl(t, 0);
-        if ((size_t)t * 64 >= L.n_items) break;
+        if ((size_t)t * 64 >= L.n_items) break;  // wave-uniform: the whole wave leaves together
         const uint32_t i = t * 64 + lane;  // n_items is a multiple of 64
         const uint32_t v = i % L.variants, p = i / L.variants;
         const uint32_t tile = p >> 6, w = p & 63u;
@@ -429,8 +437,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         if (!BACKWARD) {
             f3 sum = mk3(0.0f);
             for (uint32_t k = 0; k < L.spp; k++) {
-                const f3 c = adj_sample<false>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f), nullptr,
-                                               nullptr, stk);
+                const f3 c = adj_sample<false, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f),
+                                                      nullptr, nullptr, stk, cnt);
                 if (B.cache) {
                     float* cp = B.cache + (size_t)k * 3 * L.n_items + i;
                     cp[0] = c.x;
@@ -453,14 +461,15 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                     const float* cp = B.cache + (size_t)k * 3 * L.n_items + i;
                     c = mk3(cp[0], cp[L.n_items], cp[2 * (size_t)L.n_items]);
                 } else {
-                    c = adj_sample<false>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f), nullptr, nullptr,
-                                          stk);
+                    c = adj_sample<false, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f), nullptr,
+                                                 nullptr, stk, cnt);
                 }
                 if (isnan(luminance(c))) continue;
-                adj_sample<true>(sv, L, alb, rgh, x, y, k ^ L.seed, c, delta, ga, gr, stk);
+                adj_sample<true, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, c, delta, ga, gr, stk, cnt);
             }
         }
     }
+    if (COUNT) flush_counts(cnt, counts);
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
@@ -472,12 +481,17 @@ __global__ void k_zero_word(uint32_t* p) {
     if (threadIdx.x == 0) p[0] = 0u;
 }
 
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 }  // namespace
 
 lt_status launch_adjoint(lt_context* ctx, const AdjLaunch& L, const AdjBuffers& B, bool backward, uint32_t* work,
-                         uint32_t* spill, uint32_t blocks, hipStream_t stream) {
+                         uint32_t* spill, uint32_t blocks, unsigned long long* counts, hipStream_t stream) {
     SceneView sv = make_scene_view(ctx);
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, stream, work);
+    if (counts) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, counts, (uint32_t)kCountSlots);
     if (backward) {
         const size_t na = (size_t)L.variants * L.aw * L.ah * 3, nr = (size_t)L.variants * L.rw * L.rh;
         if (L.aid >= 0 && na)
@@ -486,9 +500,19 @@ lt_status launch_adjoint(lt_context* ctx, const AdjLaunch& L, const AdjBuffers& 
         if (L.rid >= 0 && nr)
             hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<size_t>((nr + 255) / 256, 4096)), dim3(256), 0,
                                stream, B.g_rough, nr);
-        hipLaunchKernelGGL(k_adjoint<true>, dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work, spill);
+        if (counts)
+            hipLaunchKernelGGL((k_adjoint<true, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work, spill,
+                               counts);
+        else
+            hipLaunchKernelGGL((k_adjoint<true, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work,
+                               spill, counts);
     } else {
-        hipLaunchKernelGGL(k_adjoint<false>, dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work, spill);
+        if (counts)
+            hipLaunchKernelGGL((k_adjoint<false, true>), dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work,
+                               spill, counts);
+        else
+            hipLaunchKernelGGL((k_adjoint<false, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, sv, L, B, work,
+                               spill, counts);
     }
     LT_HIP(ctx, hipGetLastError());
     return LT_OK;

@@ -20,6 +20,9 @@ struct lt_adjoint {
     // to the context's own work
     uint32_t* work = nullptr;   // persistent-grid work counter
     uint32_t* spill = nullptr;  // traversal stack spill of the resident threads
+    // counting mode: kCountSlots counters of the last forward / backward call
+    unsigned long long* counts = nullptr;
+    bool counting = false;
 };
 
 namespace {
@@ -143,6 +146,7 @@ lt_status lt_adjoint_destroy(lt_adjoint* a) {
     (void)hipDeviceSynchronize();
     if (a->work) (void)hipFree(a->work);
     if (a->spill) (void)hipFree(a->spill);
+    if (a->counts) (void)hipFree(a->counts);
     delete a;
     return LT_OK;
 }
@@ -174,7 +178,8 @@ lt_status lt_adjoint_forward(lt_adjoint* a, const lt_camera* cam, uint32_t seed,
     AdjLaunch L{};
     fill_launch(a, cam, seed, L);
     AdjBuffers B{albedo, roughness, images, sample_cache, nullptr, nullptr, nullptr};
-    return launch_adjoint(ctx, L, B, false, a->work, a->spill, a->blocks, (hipStream_t)stream);
+    return launch_adjoint(ctx, L, B, false, a->work, a->spill, a->blocks, a->counting ? a->counts : nullptr,
+                          (hipStream_t)stream);
 }
 
 lt_status lt_adjoint_backward(lt_adjoint* a, const lt_camera* cam, uint32_t seed, const float* albedo,
@@ -191,7 +196,42 @@ lt_status lt_adjoint_backward(lt_adjoint* a, const lt_camera* cam, uint32_t seed
     AdjLaunch L{};
     fill_launch(a, cam, seed, L);
     AdjBuffers B{albedo, roughness, nullptr, (float*)sample_cache, d_images, grad_albedo, grad_roughness};
-    return launch_adjoint(ctx, L, B, true, a->work, a->spill, a->blocks, (hipStream_t)stream);
+    return launch_adjoint(ctx, L, B, true, a->work, a->spill, a->blocks, a->counting ? a->counts : nullptr,
+                          (hipStream_t)stream);
+}
+
+lt_status lt_adjoint_set_counting(lt_adjoint* a, int32_t on) {
+    if (!a) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = a->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (on && !a->counts) {
+        hipError_t e = hipMalloc((void**)&a->counts, kCountSlots * sizeof(unsigned long long));
+        if (e != hipSuccess) {
+            a->counts = nullptr;
+            return lt_hip_check(ctx, e, "lt_adjoint_set_counting: hipMalloc");
+        }
+        LT_HIP(ctx, hipMemset(a->counts, 0, kCountSlots * sizeof(unsigned long long)));
+    }
+    a->counting = on != 0;
+    return LT_OK;
+}
+
+lt_status lt_adjoint_counts(lt_adjoint* a, uint64_t counts[6]) {
+    if (!a || !counts) return LT_ERR_INVALID_ARG;
+    lt_context* ctx = a->ctx;
+    if (!a->counts) return lt_fail(ctx, LT_ERR_NOT_READY, "lt_adjoint_counts: counting was never on");
+    (void)hipSetDevice(ctx->device);
+    unsigned long long h[kCountSlots];
+    LT_HIP(ctx, hipDeviceSynchronize());
+    LT_HIP(ctx, hipMemcpy(h, a->counts, sizeof(h), hipMemcpyDeviceToHost));
+    // closest rays, occlusion rays, nodes and triangles of all queries, those of the occlusion queries
+    counts[0] = h[0];
+    counts[1] = h[1];
+    counts[2] = h[2];
+    counts[3] = h[3];
+    counts[4] = h[7];
+    counts[5] = h[8];
+    return LT_OK;
 }
 
 }  // extern "C"

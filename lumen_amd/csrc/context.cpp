@@ -234,6 +234,13 @@ static lt_status validate_scene(const lt_scene_desc* s, SceneHost& h, std::strin
     }
     if (!s->index_counts && s->num_prim_infos && s->prim_infos[0].index_offset != 0)
         return bad(LT_ERR_INVALID_ARG, "prim_infos[0].index_offset must be 0");
+    // a NaN or infinite position would poison the BVH build's bounds and the
+    // Morton codes of every primitive
+    for (uint32_t v = 0; v < s->num_vertices; v++) {
+        const float* p = s->vertices[v].pos;
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]))
+            return bad(LT_ERR_INVALID_ARG, "vertices[" + std::to_string(v) + "].pos is not finite");
+    }
     // transforms: uploaded only where some mesh or area light is not the identity
     if (s->world_matrices) {
         h.xf.resize((size_t)kXfFloats * s->num_prim_infos);

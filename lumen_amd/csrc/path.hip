@@ -9,6 +9,7 @@
 
 #include "context.h"
 #include "dev_scene.h"
+#include "dev_atmosphere.h"
 #include "dev_nee.h"
 #include "dev_trace.h"
 #include "path.h"
@@ -60,7 +61,8 @@ LT_DEV bool path_bounce(const SceneView& sv, const PathLaunch& L, PathState& p, 
         Payload hp;
         trace_closest<COUNT>(sv, p.origin, p.direction, stk, hp, c);
         if (!hp.valid) {
-            if (p.depth > 0 || L.direct_lighting == 1) p.col += p.throughput * ld3(L.sky_col);
+            if (p.depth > 0 || L.direct_lighting == 1)
+                p.col += p.throughput * sky_radiance(sv, ld3(L.sky_col), p.origin, p.direction, PATH_TMAX);
             return false;
         }
         const Mat hit_mat = load_material(sv, hp.material_idx, hp.uv);

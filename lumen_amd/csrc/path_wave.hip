@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "context.h"
+#include "dev_atmosphere.h"
 #include "dev_scene.h"
 #include "dev_trace.h"
 #include "path.h"
@@ -133,7 +134,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LT_WF_SHADE
         const uint32_t x = p % L.width, y = p / L.width;
         RNG seed = {x, y, frame_word, counter};
         if (hr.y == 0xFFFFFFFFu) {
-            if (depth > 0 || L.direct_lighting == 1) c += throughput * ld3(L.sky_col);
+            if (depth > 0 || L.direct_lighting == 1) {
+                const float4 o4 = S.ray_o[p];
+                c += throughput * sky_radiance(sv, ld3(L.sky_col), mk3(o4.x, o4.y, o4.z), direction, PATH_TMAX);
+            }
         } else {
             Payload hp;
             fill_payload(sv, hr.y, hr.x, __uint_as_float(hr.z), __uint_as_float(hr.w), hp);

@@ -488,6 +488,13 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
 #ifndef LT_VCM_CELL_DIV
 #define LT_VCM_CELL_DIV 2
 #endif
+#ifndef LT_VCM_MERGE_UNROLL
+#define LT_VCM_MERGE_UNROLL 4
+#endif
+constexpr int kMergeUnroll = LT_VCM_MERGE_UNROLL;
+// pending merges per wave: at most 63 carried + 64 per unrolled photon
+constexpr uint32_t kMergeRing = 512u;
+static_assert(kMergeUnroll >= 1 && 63 + 64 * kMergeUnroll <= (int)kMergeRing, "merge ring too small");
 // COUNT: the wave tallies the photons it tests (16 B position each), the
 // photons it merges (48 B record each) and the algorithmic bytes of its
 // queries (80-B record + 8 B of bucket range per cell) into counts[4..6].
@@ -498,11 +505,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     const uint32_t lane = threadIdx.x & 63u;
     unsigned long long n_tested = 0, n_merged = 0, n_bytes = 0;  // wave-uniform
     // photons of the wave's query that passed the distance and cell tests,
-    // waiting for a full wave of merges (at most 63 + 64 pending)
-    __shared__ uint32_t s_k[kTraceBlock / 64u][128];
-    __shared__ float s_d2[kTraceBlock / 64u][128];
+    // waiting for a full wave of merges: a ring per wave (index, squared
+    // distance, the photon's d_vcm)
+    __shared__ uint32_t s_k[kTraceBlock / 64u][kMergeRing];
+    __shared__ float s_d2[kTraceBlock / 64u][kMergeRing];
+    __shared__ float s_w[kTraceBlock / 64u][kMergeRing];
     uint32_t* const wk = s_k[threadIdx.x / 64u];
     float* const wd2 = s_d2[threadIdx.x / 64u];
+    float* const ww = s_w[threadIdx.x / 64u];
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t Q = *n_queries;
     const float r2 = L.radius * L.radius;
@@ -541,13 +551,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         const int ncells = (hi[0] - lo[0] + 1) * ny * nz;
         if (COUNT) n_bytes += sizeof(VcmQuery) + 8ull * (unsigned long long)ncells;
         f3 part = mk3(0.0f);
-        uint32_t pend = 0u;  // wave-uniform
+        uint32_t head = 0u, pend = 0u;  // wave-uniform: ring position and fill
         auto merge_slot = [&](uint32_t slot) {
             const uint32_t k = wk[slot];
             const VcmPhotonRest& pr = Gr.prest[k];
             const float4 b = pr.b, c = pr.c, d = pr.d;
             f3 r;
-            if (merge_one(L, wd2[slot], f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, Gr.ppos[k].w, b.w,
+            if (merge_one(L, wd2[slot], f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, f3{d.x, d.y, d.z}, ww[slot], b.w,
                           __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
                 part += r;
         };
@@ -570,65 +580,73 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
             }
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             if (COUNT) n_tested += total;
-            for (uint32_t t0 = 0; t0 < total; t0 += 64u) {
-                const uint32_t t = t0 + lane;
-                int j = 0;  // the cell holding photon t: the first lane whose scan exceeds t
-                for (int step = 32; step > 0; step >>= 1) {
-                    const uint32_t v = __shfl(incl, j + step - 1);
-                    if (v <= t) j += step;
-                }
-                j = j < 63 ? j : 63;
-                const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
-                const int jx = __shfl(cx, j), jy = __shfl(cy, j), jz = __shfl(cz, j);
-                bool keep = false;
-                uint32_t k = 0u;
-                float dist_sqr = 0.0f;
-                if (t < total) {
-                    k = sj + (t - ej);
-                    const float4 a = Gr.ppos[k];
-                    const f3 ppos = {a.x, a.y, a.z};
-                    const f3 pp = qpos - ppos;
-                    dist_sqr = dot(pp, pp);
-                    if (dist_sqr <= r2) {
-                        int g[3];
-                        grid_idx_fast(L, ppos, g);
-                        keep = g[0] == jx && g[1] == jy && g[2] == jz;  // else another cell of the bucket
+            // kMergeUnroll photons per lane per step: their positions are
+            // loaded together, so one memory latency covers them all
+            for (uint32_t t0 = 0; t0 < total; t0 += 64u * kMergeUnroll) {
+                uint32_t kk[kMergeUnroll];
+                int jx[kMergeUnroll], jy[kMergeUnroll], jz[kMergeUnroll];
+                bool live[kMergeUnroll];
+                float4 pa[kMergeUnroll];
+#pragma unroll
+                for (int u = 0; u < kMergeUnroll; u++) {
+                    const uint32_t t = t0 + 64u * (uint32_t)u + lane;
+                    int j = 0;  // the cell holding photon t: the first lane whose scan exceeds t
+                    for (int step = 32; step > 0; step >>= 1) {
+                        const uint32_t v = __shfl(incl, j + step - 1);
+                        if (v <= t) j += step;
                     }
+                    j = j < 63 ? j : 63;
+                    const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
+                    jx[u] = __shfl(cx, j);
+                    jy[u] = __shfl(cy, j);
+                    jz[u] = __shfl(cz, j);
+                    live[u] = t < total;
+                    kk[u] = live[u] ? sj + (t - ej) : 0u;
                 }
-                const uint64_t m = __ballot(keep);
-                if (keep) {
-                    const uint32_t slot = pend + (uint32_t)__popcll(m & below);
-                    wk[slot] = k;
-                    wd2[slot] = dist_sqr;
+#pragma unroll
+                for (int u = 0; u < kMergeUnroll; u++) pa[u] = live[u] ? Gr.ppos[kk[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int u = 0; u < kMergeUnroll; u++) {
+                    bool keep = false;
+                    float dist_sqr = 0.0f;
+                    if (live[u]) {
+                        const f3 ppos = {pa[u].x, pa[u].y, pa[u].z};
+                        const f3 pp = qpos - ppos;
+                        dist_sqr = dot(pp, pp);
+                        if (dist_sqr <= r2) {
+                            int g[3];
+                            grid_idx_fast(L, ppos, g);
+                            keep = g[0] == jx[u] && g[1] == jy[u] && g[2] == jz[u];  // else another cell of the bucket
+                        }
+                    }
+                    const uint64_t m = __ballot(keep);
+                    if (keep) {
+                        const uint32_t slot = (head + pend + (uint32_t)__popcll(m & below)) & (kMergeRing - 1u);
+                        wk[slot] = kk[u];
+                        wd2[slot] = dist_sqr;
+                        ww[slot] = pa[u].w;
+                    }
+                    pend += (uint32_t)__popcll(m);
+                    if (COUNT) n_merged += (unsigned long long)__popcll(m);
                 }
-                pend += (uint32_t)__popcll(m);
-                if (COUNT) n_merged += (unsigned long long)__popcll(m);
-                if (pend >= 64u) {  // one full wave of merges, then the rest moves down
+                while (pend >= 64u) {  // full waves of merges
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    merge_slot(lane);
+                    merge_slot((head + lane) & (kMergeRing - 1u));
+                    head += 64u;
                     pend -= 64u;
-                    uint32_t mk = 0u;
-                    float md = 0.0f;
-                    if (lane < pend) {
-                        mk = wk[64u + lane];
-                        md = wd2[64u + lane];
-                    }
+                    // later appends reuse these slots only after every lane's reads
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (lane < pend) {
-                        wk[lane] = mk;
-                        wd2[lane] = md;
-                    }
                 }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane < pend) merge_slot(lane);
+        if (lane < pend) merge_slot((head + lane) & (kMergeRing - 1u));
         // the next query's writes must follow every lane's reads of this one
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

@@ -32,6 +32,23 @@ def build() -> Path:
     return LIB_PATH
 
 
+def _f3(v):
+    return np.ascontiguousarray(np.asarray(v, np.float32).reshape(3))
+
+
+def scattering(origin, direction, length, light_dir, light_col) -> np.ndarray:
+    """integrate_scattering (atmosphere.glsl:143-205) of the oracle."""
+    a = [_f3(origin), _f3(direction), _f3(light_dir), _f3(light_col)]
+    out = np.zeros(3, np.float32)
+    lib().ora_scattering(a[0].ctypes.data, a[1].ctypes.data, float(length), a[2].ctypes.data, a[3].ctypes.data,
+                         out.ctypes.data)
+    return out
+
+
+def phase(which: str, costh: float, g: float = 0.85) -> float:
+    return float(lib().ora_phase(0 if which == "rayleigh" else 1, costh, g))
+
+
 def ref_stbi_load(path: str):
     """RGBA8 image decoded by the reference's stb_image (oracle/_ref), or
     None where that library is not built."""
@@ -114,6 +131,10 @@ def lib() -> C.CDLL:
         L.ora_trace_closest.argtypes = [P, P, C.c_size_t, P, C.c_int, C.c_int]
         L.ora_trace_any.argtypes = [P, P, C.c_size_t, P, C.c_int, C.c_int]
         L.ora_payload.argtypes = [P, P, P, P]
+        L.ora_atmosphere.argtypes = [P, P, P, P, C.c_float, P]
+        L.ora_scattering.argtypes = [P, P, C.c_float, P, P, P]
+        L.ora_phase.argtypes = [C.c_int, C.c_float, C.c_float]
+        L.ora_phase.restype = C.c_float
         L.ora_render_path.argtypes = [P, P, P, C.POINTER(PathParams), P, C.c_int, C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]
         L.ora_render_path.restype = C.c_double
@@ -270,6 +291,14 @@ class Accel:
         lib().ora_trace_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, int(brute),
                             threads or lib().ora_hardware_threads())
         return occ
+
+    def atmosphere(self, sky_col, origin, direction, length) -> np.ndarray:
+        """shade_atmosphere (commons.glsl:156-168) of this scene."""
+        a = [_f3(sky_col), _f3(origin), _f3(direction)]
+        out = np.zeros(3, np.float32)
+        lib().ora_atmosphere(C.byref(self.desc), a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, float(length),
+                             out.ctypes.data)
+        return out
 
     def payload(self, hit, ray) -> np.ndarray:
         out = np.zeros(13, np.float32)

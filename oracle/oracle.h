@@ -248,6 +248,11 @@ struct Counters {
     uint64_t closest = 0, shadow = 0;
 };
 // pt_commons.glsl:3-42 at a vertex with closest-hit payload `hit`
+// sky of scenes with a directional light (oracle_atmosphere.cpp)
+float phase_rayleigh(float costh);
+float phase_mie(float costh, float g);
+vec3 integrate_scattering(vec3 start, vec3 dir, float len, vec3 ldir, vec3 Lc);
+vec3 shade_atmosphere(const lt_scene_desc& s, vec3 sky_col, vec3 o, vec3 d, float len);
 vec3 nee_uniform_sample_light(const lt_scene_desc& s, const Accel* a, uvec4& seed, const Material& mat, vec3 pos,
                               bool side, vec3 n_s, vec3 wo, const Payload& hit, Counters* cnt);
 // One sample of pixel (x, y).  `nan_dropped` reports the reference's NaN guard.

@@ -154,6 +154,23 @@ void ora_trace_any(void* a, const lt_ray* rays, size_t n, uint8_t* occ, int mode
 }
 
 // payload of a closest hit: out = [n_g(3), n_s(3), pos(3), uv(2), area, dist]
+// shade_atmosphere of scene s for the ray (o, d) of length len: rgb into out
+void ora_atmosphere(const lt_scene_desc* s, const float* sky, const float* o, const float* d, float len, float* out) {
+    const vec3 r = shade_atmosphere(*s, load3(sky), load3(o), load3(d), len);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+// integrate_scattering for an explicit light direction and colour
+void ora_scattering(const float* o, const float* d, float len, const float* ldir, const float* Lc, float* out) {
+    const vec3 r = integrate_scattering(load3(o), load3(d), len, load3(ldir), load3(Lc));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+// phase functions: which = 0 Rayleigh, 1 Mie (asymmetry g)
+float ora_phase(int which, float costh, float g) { return which == 0 ? phase_rayleigh(costh) : phase_mie(costh, g); }
+
 void ora_payload(const lt_scene_desc* s, const lt_hit* h, const lt_ray* r, float* out) {
     Hit hh{h->t, h->u, h->v, h->prim, h->inst};
     Payload p;

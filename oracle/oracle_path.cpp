@@ -117,7 +117,8 @@ vec3 path_sample(const lt_scene_desc& s, const Accel* a, const lt_camera& cam, c
     for (int depth = 0;; depth++) {
         trace_closest(s, a, origin, direction, PATH_TMIN, PATH_TMAX, st, cnt);
         if (!st.hit_valid) {
-            if (depth > 0 || pp.direct_lighting == 1) col += throughput * pp.sky_col;
+            if (depth > 0 || pp.direct_lighting == 1)
+                col += throughput * shade_atmosphere(s, pp.sky_col, origin, direction, PATH_TMAX);  // path.rgen:50-53
             break;
         }
         const Payload hp = st.pl;

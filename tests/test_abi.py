@@ -187,3 +187,12 @@ def test_upload_validation_refusals():
     d.lights[0].num_triangles = 3  # the instance's light claims 3 triangles of 2
     st, msg = _validate(d)
     assert st == 1 and "triangle count" in msg
+    d.lights[0].num_triangles = 2
+    assert _validate(d) == (0, "")
+    for bad in (float("nan"), float("inf")):  # a poisoned vertex position
+        old = d.vertices[2].pos[1]
+        d.vertices[2].pos[1] = bad
+        st, msg = _validate(d)
+        assert st == 1 and "vertices[2].pos" in msg
+        d.vertices[2].pos[1] = old
+    assert _validate(d) == (0, "")

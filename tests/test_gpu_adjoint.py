@@ -112,6 +112,42 @@ def test_adjoint_albedo_central_differences_on_gpu(gpu_ctx):
     adj.close()
 
 
+def test_adjoint_counting_variants(gpu_ctx):
+    """The counting kernels (lt_adjoint_set_counting) give the same images bit
+    for bit and the same gradients up to the order of their float atomics,
+    and count the work: at least one camera ray per
+    sample, at least one node per ray; the cached replay traces exactly the
+    forward's closest-hit rays (every pixel has a non-zero adjoint) and at
+    least its occlusion rays (it also tests lights whose contribution is zero
+    but whose derivative is not)."""
+    print("[gpu] adjoint counting variants", flush=True)
+    from lumen_amd.adjoint import Adjoint
+    W, H, V, spp = 24, 16, 2, 2
+    sc, alb, rough = _scene()
+    albs, roughs = _variants(alb, rough, V, np.random.default_rng(3))
+    w = torch.from_numpy(np.random.default_rng(4).normal(size=(V, H, W, 3)).astype(np.float32)).cuda()
+    adj = Adjoint(sc, gpu_ctx, W, H, spp=spp, variants=V, albedo_texture=0, roughness_texture=1, max_depth=6)
+    ta, tr = torch.from_numpy(albs).cuda(), torch.from_numpy(roughs).cuda()
+    img, cache = adj.forward(ta, tr, seed=3)
+    ga, gr = adj.backward(ta, tr, w, seed=3, cache=cache)
+    adj.set_counting(True)
+    img_c, cache_c = adj.forward(ta, tr, seed=3)
+    cf = adj.counts()
+    ga_c, gr_c = adj.backward(ta, tr, w, seed=3, cache=cache_c)
+    cb = adj.counts()
+    adj.set_counting(False)
+    torch.cuda.synchronize()
+    assert torch.equal(img, img_c) and torch.equal(cache, cache_c)
+    _cmp_grad(ga_c.cpu().numpy(), ga.cpu().numpy(), "albedo gradient, counting variant")
+    _cmp_grad(gr_c.cpu().numpy(), gr.cpu().numpy(), "roughness gradient, counting variant")
+    print(f"[gpu] adjoint counts forward {cf}, backward {cb}", flush=True)
+    assert cf["closest"] >= V * W * H * spp and cf["nodes"] >= cf["closest"] + cf["shadow"]
+    assert cf["tris"] > 0 and cf["shadow"] > 0 and cf["shadow_nodes"] <= cf["nodes"]
+    assert cb["closest"] == cf["closest"] and cb["tris"] - cb["shadow_tris"] == cf["tris"] - cf["shadow_tris"]
+    assert cb["nodes"] - cb["shadow_nodes"] == cf["nodes"] - cf["shadow_nodes"] and cb["shadow"] >= cf["shadow"]
+    adj.close()
+
+
 def test_autograd_function_matches_direct_backward(gpu_ctx):
     print("[gpu] adjoint autograd", flush=True)
     from lumen_amd.adjoint import Adjoint, render_textures

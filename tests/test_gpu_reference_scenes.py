@@ -1,5 +1,7 @@
 """The reference's own scenes (tests/golden/scenes: scenes/cornell_box with its
-Path, BDPT, VCM and SPPM integrator blocks, and scenes/caustics), loaded
+Path, BDPT, VCM and SPPM integrator blocks and its directional-light Path
+variant with the wood-textured floor and the atmosphere sky, and
+scenes/caustics), loaded
 through lt_scene_load_json and rendered by the integrator their JSON names
 on the GPU, against the oracle on the same scene, camera, seed and frames.
 
@@ -26,11 +28,13 @@ FIX = Path(__file__).resolve().parent / "golden" / "scenes"
 def _load(tmp_path, json_name, obj_name):
     (tmp_path / obj_name).write_bytes(gzip.decompress((FIX / (obj_name + ".gz")).read_bytes()))
     (tmp_path / json_name).write_bytes((FIX / json_name).read_bytes())
+    (tmp_path / "wood1.jpg").write_bytes((FIX / "wood1.jpg").read_bytes())
     return la.Scene.load_json(str(tmp_path / json_name))
 
 
 @pytest.mark.parametrize("json_name,obj_name,frames", [
     ("cornell_box_path.json", "cornell_box.obj", 3),
+    ("cornell_box_dir.json", "cornell_box.obj", 3),
     ("cornell_box_bdpt.json", "cornell_box.obj", 2),
     ("cornell_box_vcm.json", "cornell_box.obj", 2),
     ("cornell_box_sppm.json", "cornell_box.obj", 3),

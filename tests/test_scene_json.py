@@ -167,3 +167,29 @@ def test_reference_caustics_scene():
     assert cfg.kind == 2 and cfg.path_length == 10 and cfg.enable_vm == 1
     _, _, lights, _ = _arrays(sc)
     assert any((l.light_flags & 7) == 1 for l in lights)  # the spot light
+
+
+def test_reference_directional_scene_with_texture(tmp_path):
+    """scenes/cornell_box/cornell_box_dir.json (a fixture with its mesh and
+    wood1.jpg): the floor's "texture" becomes texture 0, 259 x 194 linear
+    texels decoded from the JPEG through the sRGB curve; the directional
+    light sets dir_light_idx; the Path block's grey sky."""
+    import gzip
+    fix = Path(__file__).resolve().parent / "golden" / "scenes"
+    (tmp_path / "cornell_box.obj").write_bytes(gzip.decompress((fix / "cornell_box.obj.gz").read_bytes()))
+    for f in ("cornell_box_dir.json", "wood1.jpg"):
+        (tmp_path / f).write_bytes((fix / f).read_bytes())
+    sc = la.Scene.load_json(str(tmp_path / "cornell_box_dir.json"))
+    mats, prims, lights, _ = _arrays(sc)
+    d = sc.desc
+    assert d.num_textures == 1 and d.textures[0].width == 259 and d.textures[0].height == 194
+    floor = [m for m in mats if m.texture_id == 0]
+    assert len(floor) == 1 and np.allclose(floor[0].albedo[:], [0.725, 0.71, 0.68])
+    tex = np.ctypeslib.as_array(d.textures[0].data, shape=(194 * 259 * 4,)).reshape(194, 259, 4)
+    rgba8 = la.load_image_rgba8(fix / "wood1.jpg")
+    c = rgba8[..., :3] / 255.0
+    lin = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+    assert np.allclose(tex[..., :3], lin, rtol=1e-6, atol=1e-7) and np.all(tex[..., 3] == 1.0)
+    assert d.dir_light_idx >= 0 and (lights[d.dir_light_idx].light_flags & 7) == 3
+    cfg = sc.integrator_cfg()
+    assert cfg.kind == 0 and cfg.path_length == 6 and np.allclose(cfg.sky_col[:], 0.5)
