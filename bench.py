@@ -355,6 +355,9 @@ def bench_adjoint(args) -> int:
     roof = line("k_adjoint<true> (path replay with texel-gradient atomics)", bwd_bytes, bwd_ms, cb)
     roof["other_kernels"] = {"k_adjoint<false>": line("k_adjoint<false> (forward, writes the radiance cache)",
                                                       fwd_bytes, fwd_ms, cf)}
+    roof["traffic"] = traffic_from_profile(f"diff-cornell-wall16k-tex256-{W}x{H}-spp{spp}-v{V}-d{depth}", "adjoint")
+    roof["note"] = ("the scene's BVH8 (16k-triangle wall and box) stays in L2 and the Infinity Cache, so the "
+                    "algorithmic bytes can exceed the HBM peak; traffic is what reaches HBM")
     log(f"adjoint roofline: {roof}")
     cpu = None
     if not args.no_cpu:

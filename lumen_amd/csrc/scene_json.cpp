@@ -445,10 +445,12 @@ bool make_material(const JVal& b, lt_material& m, const std::string& dir, lt_hos
         if (m.roughness < 1.0f) m.bsdf_props |= LT_BSDF_FLAG_REFLECTION;
         if (m.spec_trans > 0.0f) m.bsdf_props |= LT_BSDF_FLAG_TRANSMISSION;
         m.bsdf_props |= (double)m.roughness > 0.08 ? LT_BSDF_FLAG_GLOSSY : LT_BSDF_FLAG_SPECULAR;
-    } else {
-        err = "unknown bsdf type '" + type + "'";
-        return false;
     }
+    // Any other type (e.g. "disney" in scenes/cornell_box/cornell_box_disney.json)
+    // falls through the reference's chain of type tests (LumenScene.cpp:372-475)
+    // and keeps the zero-initialised bsdf_type and bsdf_props with the albedo,
+    // emission and texture read above: a surface that emits but never scatters
+    // (every BSDF switch of the shaders leaves f and the pdf at zero for it).
     return true;
 }
 

@@ -492,8 +492,10 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
 #define LT_VCM_MERGE_UNROLL 4
 #endif
 constexpr int kMergeUnroll = LT_VCM_MERGE_UNROLL;
-// pending merges per wave: at most 63 carried + 64 per unrolled photon
-constexpr uint32_t kMergeRing = 512u;
+// pending merges per wave: at most 63 carried + 64 per unrolled photon,
+// rounded up to a power of two (512 for the default of four)
+constexpr uint32_t ring_size(uint32_t need) { return need <= 1u ? 1u : 2u * ring_size((need + 1u) / 2u); }
+constexpr uint32_t kMergeRing = ring_size(63u + 64u * (uint32_t)kMergeUnroll);
 static_assert(kMergeUnroll >= 1 && 63 + 64 * kMergeUnroll <= (int)kMergeRing, "merge ring too small");
 // COUNT: the wave tallies the photons it tests (16 B position each), the
 // photons it merges (48 B record each) and the algorithmic bytes of its

@@ -1,13 +1,17 @@
 """The reference's own scenes (tests/golden/scenes: scenes/cornell_box with its
 Path, BDPT, VCM and SPPM integrator blocks and its directional-light Path
-variant with the wood-textured floor and the atmosphere sky, and
-scenes/caustics), loaded
-through lt_scene_load_json and rendered by the integrator their JSON names
-on the GPU, against the oracle on the same scene, camera, seed and frames.
+variant with the wood-textured floor and the atmosphere sky and its variant
+with an unknown ("disney") BSDF type, scenes/caustics and its zoomed view,
+the occluded-light VCM scenes occluded and occluded2, and the material test
+scene material_test/materials with its dielectrics, conductors and principled
+materials), loaded through lt_scene_load_json and rendered by the integrator
+their JSON names on the GPU, against the oracle on the same scene, camera,
+seed and frames.
 
-The scenes come from the reference unchanged: an 8-shape Cornell box of
-1,992 triangles with a glass sphere, a 9,733-triangle caustics scene with a
-spot light.  The bar is the one of the integrators' replays (per-pixel
+The scenes come from the reference unchanged: an 8-shape Cornell box of 72
+triangles and 1,920 quads (3,912 triangles) with a glass sphere, an
+18,893-triangle caustics scene with a spot light, the material test scene of
+23,112 triangles.  The bar is the one of the integrators' replays (per-pixel
 |gpu - cpu| <= 1e-3 (1 + |cpu|) on >= 99.5 % of the pixels, relative RMSE
 <= 1 %)."""
 import gzip
@@ -39,6 +43,11 @@ def _load(tmp_path, json_name, obj_name):
     ("cornell_box_vcm.json", "cornell_box.obj", 2),
     ("cornell_box_sppm.json", "cornell_box.obj", 3),
     ("caustics.json", "caustics.obj", 2),
+    ("cornell_box_disney.json", "cornell_box.obj", 3),
+    ("caustics_zoomed.json", "caustics.obj", 2),
+    ("occluded.json", "occluded.obj", 2),
+    ("occluded2.json", "occluded2.obj", 2),
+    ("materials.json", "materials.obj", 3),
 ])
 def test_reference_scene_renders_like_the_oracle(gpu_ctx, tmp_path, json_name, obj_name, frames):
     sc = _load(tmp_path, json_name, obj_name)

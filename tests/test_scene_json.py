@@ -127,12 +127,25 @@ def test_sppm_integrator_block(tmp_path):
     ("{\"integrator\": {\"type\": \"path\"}, \"bsdfs\": [], ", "LT_ERR_PARSE"),
     ("{\"bsdfs\": [{\"type\": \"diffuse\", \"texture\": \"a.png\", \"refs\": []}], \"mesh_file\": \"scene.obj\"}",
      "LT_ERR_PARSE"),
-    ("{\"bsdfs\": [{\"type\": \"velvet\", \"refs\": []}], \"mesh_file\": \"scene.obj\"}", "LT_ERR_PARSE"),
     ("{\"bsdfs\": [{\"type\": \"diffuse\", \"refs\": []}], \"mesh_file\": \"missing.obj\"}", "LT_ERR_PARSE"),
 ])
 def test_errors(tmp_path, bad, code):
     with pytest.raises(LtError, match=code):
         la.Scene.load_json(str(_write(tmp_path, js=bad)))
+
+
+def test_unknown_bsdf_type_falls_through(tmp_path):
+    """A type the reference does not know ("disney" in
+    scenes/cornell_box/cornell_box_disney.json) passes its chain of type tests
+    untouched: bsdf_type and bsdf_props stay 0, the albedo and emission are
+    kept (LumenScene.cpp:361-475)."""
+    js = JSON.replace('"type": "principled", "roughness": 0.05', '"type": "velvet", "albedo": [0.2, 0.3, 0.4]')
+    assert js != JSON
+    sc = la.Scene.load_json(str(_write(tmp_path, js=js)))
+    mats, _, _, _ = _arrays(sc)
+    velvet = [m for m in mats if m.bsdf_type == 0]
+    assert len(velvet) == 1 and velvet[0].bsdf_props == 0
+    assert np.allclose(velvet[0].albedo[:], [0.2, 0.3, 0.4])
 
 
 def test_missing_file(tmp_path):
