@@ -7,13 +7,15 @@ this module, and only as the checker / the timed CPU baseline; the product
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "_build" / "liboracle.so"
+# ORACLE_LIB selects another build of the same sources (e.g. _build/liboracle_fma.so)
+LIB_PATH = Path(os.environ.get("ORACLE_LIB") or (HERE / "_build" / "liboracle.so"))
 
 
 REF_TREE = Path("/root/reference")

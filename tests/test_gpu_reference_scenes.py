@@ -13,7 +13,13 @@ triangles and 1,920 quads (3,912 triangles) with a glass sphere, an
 18,893-triangle caustics scene with a spot light, the material test scene of
 23,112 triangles.  The bar is the one of the integrators' replays (per-pixel
 |gpu - cpu| <= 1e-3 (1 + |cpu|) on >= 99.5 % of the pixels, relative RMSE
-<= 1 %)."""
+<= 1 %).  The VCM scenes also render with the corrected estimator at that
+bar.  The zoomed caustic (depth 13, the glass sphere filling the view) is
+the exception: its eye and light paths refract through the sphere so many
+times that last-bit arithmetic flips discrete choices; the oracle built with
+fused multiply-adds differs from itself there on 0.6 % of the pixels with a
+relative RMSE of 2 % (tests/test_oracle_sensitivity.py), so the device is
+held to >= 99 % of the pixels and a relative RMSE <= 2.5 % on that scene."""
 import gzip
 from pathlib import Path
 
@@ -22,11 +28,20 @@ import pytest
 
 import lumen_amd as la
 import oracle
-from test_gpu_bdpt import _compare
 
 pytestmark = pytest.mark.gpu
 
 FIX = Path(__file__).resolve().parent / "golden" / "scenes"
+
+
+def _compare_frac(g, o, tag, frac=0.995, max_rel=0.01):
+    d = np.abs(g[..., :3] - o[..., :3])
+    ok = np.all(d <= 1e-3 * (1.0 + np.abs(o[..., :3])), axis=-1)
+    rel = np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(o[..., :3] ** 2)), 1e-6)
+    print(f"[gpu] {tag}: pixels matching {ok.mean():.4f} ({int((~ok).sum())} differ), rel rmse {rel:.2e}",
+          flush=True)
+    assert np.all(np.isfinite(g))
+    assert ok.mean() >= frac and rel <= max_rel
 
 
 def _load(tmp_path, json_name, obj_name):
@@ -79,4 +94,18 @@ def test_reference_scene_renders_like_the_oracle(gpu_ctx, tmp_path, json_name, o
         o, _, _, _, _ = acc.render_sppm(it.camera, W, H, depth, frames=frames, seed=11, base_radius=it.base_radius,
                                         sky_col=tuple(it.sky_col))
     assert np.abs(o[..., :3]).max() > 0
-    _compare(g, o, f"reference {json_name}")
+    bar = dict(frac=0.99, max_rel=0.025) if json_name == "caustics_zoomed.json" else {}
+    _compare_frac(g, o, f"reference {json_name}", **bar)
+    if name == "vcm":
+        it = la.create_integrator(name, sc, gpu_ctx, W, H, estimator="corrected")
+        it.seed = 11
+        it.init()
+        it.create_accel()
+        for _ in range(frames):
+            it.render()
+            it.update()
+        g = it.output_tex
+        it.destroy()
+        o, _, _, _ = acc.render_vcm(it.camera, W, H, depth, frames=frames, seed=11, radius_factor=it.radius_factor,
+                                    alpha=it.alpha, use_vm=it.enable_vm, corrected=True, sky_col=tuple(it.sky_col))
+        _compare_frac(g, o, f"reference {json_name}, corrected estimator", **bar)
