@@ -421,32 +421,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
     const uint32_t lane = threadIdx.x & 63u;
     const size_t na = (size_t)L.aw * L.ah, nr = (size_t)L.rw * L.rh, npx = (size_t)L.width * L.height;
     const float inv_spp = 1.0f / (float)L.spp;
-    // Variant bands per XCD: workgroups are dealt to the 8 XCDs round-robin
-    // (blockIdx mod 8), each with its own L2.  When the variants split evenly,
-    // XCD x first takes the items of variants [x V/8, (x+1) V/8), so that its
-    // L2 holds those variants' texels and gradient texels (an eighth of the
-    // 33 MB of 32 variants of 256^2 albedo and roughness) instead of all of
-    // them; a wave whose band is used up helps the next band.  Within a band
-    // a wave's 64 lanes are 64 / (V/8) pixels times the band's variants, the
-    // variants of one pixel in adjacent lanes.
-    const uint32_t n_bands = ((gridDim.x & 7u) == 0u && (L.variants & 7u) == 0u) ? 8u : 1u;
-    const uint32_t vb = L.variants / n_bands;         // variants per band
-    const uint32_t band_items = L.n_items / n_bands;  // a multiple of 64 (tiles * 64 * vb)
-    uint32_t band = n_bands == 8u ? (blockIdx.x & 7u) : 0u, bands_done = 0u;  // wave-uniform
     for (;;) {
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(work + band, 1u);
+        if (lane == 0) t = atomicAdd(work, 1u);
         t = __shfl(t, 0);
-        if ((size_t)t * 64 >= band_items) {  // wave-uniform: the whole wave moves on together
-            if (++bands_done == n_bands) break;
-            band = band + 1u == n_bands ? 0u : band + 1u;
-            continue;
-        }
-        const uint32_t j = t * 64 + lane;
-        const uint32_t v = band * vb + j % vb, p = j / vb;
-        // the item's slot in the radiance cache: band-major, so that a wave's
-        // 64 samples are 64 consecutive floats per channel
-        const uint32_t i = band * band_items + j;
+        if ((size_t)t * 64 >= L.n_items) break;  // wave-uniform: the whole wave leaves together
+        const uint32_t i = t * 64 + lane;  // n_items is a multiple of 64
+        const uint32_t v = i % L.variants, p = i / L.variants;
         const uint32_t tile = p >> 6, w = p & 63u;
         const uint32_t x = (tile % L.tiles_x) * 8 + (w & 7u), y = (tile / L.tiles_x) * 8 + (w >> 3);
         if (x >= L.width || y >= L.height) continue;
@@ -496,9 +477,8 @@ __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
         p[i] = 0.0f;
 }
 
-// the work counters of the 8 variant bands
 __global__ void k_zero_word(uint32_t* p) {
-    if (threadIdx.x < 8u) p[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) p[0] = 0u;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {

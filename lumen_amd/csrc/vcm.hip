@@ -563,11 +563,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                           __float_as_uint(c.w), 1.0f, depth, qn, qwo, mat, side, cs, norm, r))
                 part += r;
         };
-        // this lane's cell of a chunk of 64 and its photon range; the next
-        // chunk's ranges are loaded while the current chunk's photons stream
-        auto cell_range = [&](int ci, uint32_t& st, uint32_t& cnt, int& cx, int& cy, int& cz) {
-            st = cnt = 0u;
-            cx = cy = cz = 0;
+        for (int cbase = 0; cbase < ncells; cbase += 64) {
+            const int ci = cbase + (int)lane;  // this lane's cell of the chunk: its photon range
+            uint32_t st = 0u, cnt = 0u;
+            int cx = 0, cy = 0, cz = 0;
             if (ci < ncells) {
                 cx = lo[0] + ci / (ny * nz);
                 cy = lo[1] + (ci / nz) % ny;
@@ -576,14 +575,6 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                 st = Gr.start[h];
                 cnt = Gr.start[h + 1] - st;
             }
-        };
-        uint32_t st_n, cnt_n;
-        int cx_n, cy_n, cz_n;
-        cell_range((int)lane, st_n, cnt_n, cx_n, cy_n, cz_n);
-        for (int cbase = 0; cbase < ncells; cbase += 64) {
-            const uint32_t st = st_n, cnt = cnt_n;
-            const int cx = cx_n, cy = cy_n, cz = cz_n;
-            if (cbase + 64 < ncells) cell_range(cbase + 64 + (int)lane, st_n, cnt_n, cx_n, cy_n, cz_n);
             uint32_t incl = cnt;  // inclusive scan over the wave
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t v = __shfl_up(incl, (unsigned)o);
