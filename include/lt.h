@@ -225,8 +225,15 @@ lt_status lt_scene_validate(const lt_scene_desc* scene, char* err, size_t err_le
 typedef struct lt_build_opts {
     uint32_t max_leaf_tris; /* triangles per BVH8 leaf child, 1..4 (0: the default, 1) */
     uint32_t collapse;      /* LT_COLLAPSE_SAH (0: the default) or LT_COLLAPSE_GREEDY */
-    uint32_t reserved[6];
+    uint32_t binary_tree;   /* LT_TREE_PLOC (0: the default) or LT_TREE_LBVH */
+    uint32_t reserved[5];
 } lt_build_opts;
+/* The binary tree the collapse starts from: PLOC, parallel locally-ordered
+ * clustering of the Morton-sorted triangles by the surface area of merged
+ * boxes (Meister and Bittner 2018, radius 16), or the LBVH of the Morton
+ * codes' radix tree (Karras 2012). */
+#define LT_TREE_PLOC 1u
+#define LT_TREE_LBVH 2u
 /* Collapse of the binary LBVH into BVH8 nodes: the SAH-optimal choice of each
  * node's children by dynamic programming over the binary tree (Ylitie, Karras
  * and Laine 2017, node cost 1, triangle cost 0.5), or the greedy expansion of

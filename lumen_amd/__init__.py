@@ -146,10 +146,13 @@ class Context:
         check(lib().lt_scene_upload(self._h, C.byref(scene.desc)), self._h, "lt_scene_upload")
         self.scene = scene
 
-    def build_accel(self, max_leaf_tris: int = 1, collapse: str = "sah") -> _abi.lt_build_stats:
+    def build_accel(self, max_leaf_tris: int = 1, collapse: str = "sah", tree: str = "ploc") -> _abi.lt_build_stats:
+        """Integrator::create_accel: the binary tree (PLOC clustering or the
+        Morton-code LBVH) collapsed into BVH8 nodes (SAH-optimal or greedy)."""
         o = _abi.lt_build_opts()
         o.max_leaf_tris = max_leaf_tris
         o.collapse = {"sah": _abi.LT_COLLAPSE_SAH, "greedy": _abi.LT_COLLAPSE_GREEDY}[collapse]
+        o.binary_tree = {"ploc": _abi.LT_TREE_PLOC, "lbvh": _abi.LT_TREE_LBVH}[tree]
         s = _abi.lt_build_stats()
         check(lib().lt_accel_build(self._h, C.byref(o), C.byref(s)), self._h, "lt_accel_build")
         return s

@@ -37,6 +37,7 @@ LT_MIS_BALANCE, LT_MIS_POWER = 0, 1
 LT_VCM_REFERENCE, LT_VCM_CORRECTED = 0, 1
 LT_WF_SORT_NONE, LT_WF_SORT_BSDF = 1, 2
 LT_COLLAPSE_SAH, LT_COLLAPSE_GREEDY = 1, 2
+LT_TREE_PLOC, LT_TREE_LBVH = 1, 2
 LT_XCD_BANDS_OFF, LT_XCD_BANDS_ON = 1, 2
 LT_SCENE_CORNELL, LT_SCENE_TERRAIN, LT_SCENE_CAUSTICS, LT_SCENE_DIFF_WALL = 0, 1, 2, 3
 
@@ -89,7 +90,8 @@ class lt_camera(C.Structure):
 
 
 class lt_build_opts(C.Structure):
-    _fields_ = [("max_leaf_tris", C.c_uint32), ("collapse", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("max_leaf_tris", C.c_uint32), ("collapse", C.c_uint32), ("binary_tree", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class lt_build_stats(C.Structure):

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <cmath>
 #include <vector>
 
@@ -218,6 +219,177 @@ __device__ __forceinline__ Box tri_box(const TriRec& r) {
 __device__ __forceinline__ float half_area(const Box& b) {
     float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
     return dx * dy + dy * dz + dz * dx;
+}
+
+// ---- PLOC binary tree (Meister and Bittner 2018, "Parallel locally-ordered
+// clustering for bounding volume hierarchy construction") ----
+// The clusters start as the Morton-sorted triangles.  Each round, every
+// cluster finds the neighbour within kPlocRadius positions whose union box has
+// the smallest surface area; mutual nearest neighbours merge into a new
+// internal node (allocated from n - 2 down, so the last merge is the root 0
+// and a parent's index is below its children's), and the survivors are
+// compacted in order.  Ties are broken by the pair's indices, a strict order
+// under which the globally closest pair is mutual, so every round merges.
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+    Box r;
+    for (int k = 0; k < 3; k++) {
+        r.lo[k] = fminf(a.lo[k], b.lo[k]);
+        r.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    return r;
+}
+
+__global__ void k_ploc_init(const TriRec* __restrict__ tris, const uint32_t* __restrict__ sorted, uint32_t n,
+                            uint32_t* __restrict__ cl_node, Box* __restrict__ cl_box) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cl_node[i] = 0x80000000u | i;
+    cl_box[i] = tri_box(tris[sorted[i]]);
+}
+
+// nearest neighbour of each of the m clusters within the radius: (area, lower
+// index, higher index) lexicographically smallest
+__global__ void k_ploc_nn(const Box* __restrict__ cl_box, uint32_t m, uint32_t* __restrict__ nn) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const Box bi = cl_box[i];
+    const uint32_t lo = i > (uint32_t)kPlocRadius ? i - kPlocRadius : 0u;
+    const uint32_t hi = min(m - 1u, i + (uint32_t)kPlocRadius);
+    float best = INFINITY;
+    uint32_t bj = i, bmin = 0xFFFFFFFFu, bmax = 0xFFFFFFFFu;
+    for (uint32_t j = lo; j <= hi; j++) {
+        if (j == i) continue;
+        const float a = half_area(box_union(bi, cl_box[j]));
+        const uint32_t pmin = min(i, j), pmax = max(i, j);
+        if (a < best || (a == best && (pmin < bmin || (pmin == bmin && pmax < bmax)))) {
+            best = a;
+            bj = j;
+            bmin = pmin;
+            bmax = pmax;
+        }
+    }
+    nn[i] = bj;
+}
+
+// mutual pairs merge at the lower index; keep[i] = 1 for the survivors
+__global__ void k_ploc_merge(const uint32_t* __restrict__ cl_node, const Box* __restrict__ cl_box, uint32_t m,
+                             const uint32_t* __restrict__ nn, uint32_t n, uint32_t* __restrict__ ctr,
+                             uint32_t* __restrict__ left, uint32_t* __restrict__ right,
+                             uint32_t* __restrict__ parent_int, uint32_t* __restrict__ parent_leaf,
+                             uint32_t* __restrict__ out_node, Box* __restrict__ out_box, uint32_t* __restrict__ keep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = nn[i];
+    const bool mutual = j != i && nn[j] == i;
+    if (!mutual) {
+        out_node[i] = cl_node[i];
+        out_box[i] = cl_box[i];
+        keep[i] = 1u;
+        return;
+    }
+    if (i > j) {  // merged by its partner
+        keep[i] = 0u;
+        return;
+    }
+    const uint32_t idx = n - 2u - atomicAdd(ctr, 1u);
+    const uint32_t a = cl_node[i], b = cl_node[j];
+    left[idx] = a;
+    right[idx] = b;
+    if (a & 0x80000000u)
+        parent_leaf[a & 0x7fffffffu] = idx;
+    else
+        parent_int[a] = idx;
+    if (b & 0x80000000u)
+        parent_leaf[b & 0x7fffffffu] = idx;
+    else
+        parent_int[b] = idx;
+    out_node[i] = idx;
+    out_box[i] = box_union(cl_box[i], cl_box[j]);
+    keep[i] = 1u;
+}
+
+__global__ void k_ploc_compact(const uint32_t* __restrict__ in_node, const Box* __restrict__ in_box, uint32_t m,
+                               const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                               uint32_t* __restrict__ out_node, Box* __restrict__ out_box) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || !keep[i]) return;
+    out_node[pos[i]] = in_node[i];
+    out_box[pos[i]] = in_box[i];
+}
+
+// PLOC's subtrees are not contiguous runs of the Morton order; the collapse
+// wants every subtree's triangles contiguous.  Leaf counts bottom-up (second
+// arrival sums), then each node's first position is the sum, over the
+// ancestors it lies to the right of, of their left subtrees' counts.
+__global__ void k_tree_count(const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                             const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
+                             uint32_t n, uint32_t* __restrict__ flags, uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = parent_leaf[i];
+    for (;;) {
+        if (atomicAdd(&flags[p], 1u) == 0u) return;
+        __threadfence();
+        const uint32_t a = left[p], b = right[p];
+        const uint32_t ca = (a & 0x80000000u) ? 1u : ((const volatile uint32_t*)cnt)[a];
+        const uint32_t cb = (b & 0x80000000u) ? 1u : ((const volatile uint32_t*)cnt)[b];
+        cnt[p] = ca + cb;
+        __threadfence();
+        if (p == 0u) return;
+        p = parent_int[p];
+    }
+}
+
+__device__ __forceinline__ uint32_t first_pos(uint32_t code, const uint32_t* __restrict__ left,
+                                              const uint32_t* __restrict__ right,
+                                              const uint32_t* __restrict__ parent_int,
+                                              const uint32_t* __restrict__ parent_leaf,
+                                              const uint32_t* __restrict__ cnt) {
+    uint32_t pos = 0u;
+    uint32_t c = code;
+    uint32_t p = (c & 0x80000000u) ? parent_leaf[c & 0x7fffffffu] : (c == 0u ? 0xFFFFFFFFu : parent_int[c]);
+    while (p != 0xFFFFFFFFu) {
+        if (right[p] == c) {
+            const uint32_t a = left[p];
+            pos += (a & 0x80000000u) ? 1u : cnt[a];
+        }
+        c = p;
+        p = p == 0u ? 0xFFFFFFFFu : parent_int[p];
+    }
+    return pos;
+}
+
+// internal nodes: their leaf range; leaves: their position in the new order
+__global__ void k_tree_ranges(const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                              const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
+                              const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ range_lo,
+                              uint32_t* __restrict__ range_hi, uint32_t* __restrict__ leaf_pos) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n - 1u) {
+        const uint32_t lo = first_pos(i, left, right, parent_int, parent_leaf, cnt);
+        range_lo[i] = lo;
+        range_hi[i] = lo + cnt[i] - 1u;
+    }
+    if (i < n) leaf_pos[i] = first_pos(0x80000000u | i, left, right, parent_int, parent_leaf, cnt);
+}
+
+// leaves renumbered to their new positions (children, triangle order, parents)
+__global__ void k_tree_relabel(uint32_t* __restrict__ left, uint32_t* __restrict__ right,
+                               const uint32_t* __restrict__ leaf_pos, const uint32_t* __restrict__ sorted_in,
+                               const uint32_t* __restrict__ parent_leaf_in, uint32_t n,
+                               uint32_t* __restrict__ sorted_out, uint32_t* __restrict__ parent_leaf_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n - 1u) {
+        const uint32_t a = left[i], b = right[i];
+        if (a & 0x80000000u) left[i] = 0x80000000u | leaf_pos[a & 0x7fffffffu];
+        if (b & 0x80000000u) right[i] = 0x80000000u | leaf_pos[b & 0x7fffffffu];
+    }
+    if (i < n) {
+        sorted_out[leaf_pos[i]] = sorted_in[i];
+        parent_leaf_out[leaf_pos[i]] = parent_leaf_in[i];
+    }
 }
 
 // SAH weights of a BVH8 node visit and of a triangle test (a watertight
@@ -517,6 +689,8 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
     if (collapse != LT_COLLAPSE_SAH && collapse != LT_COLLAPSE_GREEDY)
         return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown collapse");
     const bool sah = collapse == LT_COLLAPSE_SAH;
+    const uint32_t tree = opts && opts->binary_tree ? opts->binary_tree : LT_TREE_PLOC;
+    if (tree != LT_TREE_PLOC && tree != LT_TREE_LBVH) return lt_fail(ctx, LT_ERR_INVALID_ARG, "unknown binary tree");
     hipStream_t st = ctx->stream;
     DeviceBVH& bvh = ctx->bvh;
     if (bvh.nodes) (void)hipFree(bvh.nodes);
@@ -540,6 +714,12 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
     size_t sort_bytes = 0;
     float* sah_R = nullptr;
     uint8_t* sah_dec = nullptr;
+    // PLOC temporaries
+    uint32_t *cl_node0 = nullptr, *cl_node1 = nullptr, *nn = nullptr, *keep = nullptr, *kpos = nullptr,
+             *cnt = nullptr, *leaf_pos = nullptr, *vals3 = nullptr, *pleaf3 = nullptr, *mctr = nullptr;
+    Box *cl_box0 = nullptr, *cl_box1 = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
     lt_status rc = LT_OK;
     auto fail = [&](hipError_t e, const char* what) {
         rc = lt_hip_check(ctx, e, what);
@@ -588,10 +768,62 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
         BCHK(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, n, 0u, 30u, st));
         BCHK(hipMalloc(&sort_tmp, sort_bytes));
         BCHK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, 30u, st));
-        if (n > 1) {
+        if (n > 1 && tree == LT_TREE_LBVH) {
             hipLaunchKernelGGL(k_karras, dim3((n - 1 + B - 1) / B), dim3(B), 0, st, keys2, (int)n, left, right, pint,
                                pleaf, rlo, rhi);
             BCHK(hipGetLastError());
+        }
+        if (n > 1 && tree == LT_TREE_PLOC) {
+            BCHK(hipMalloc(&cl_node0, 4ull * n));
+            BCHK(hipMalloc(&cl_node1, 4ull * n));
+            BCHK(hipMalloc(&cl_box0, sizeof(Box) * n));
+            BCHK(hipMalloc(&cl_box1, sizeof(Box) * n));
+            BCHK(hipMalloc(&nn, 4ull * n));
+            BCHK(hipMalloc(&keep, 4ull * n));
+            BCHK(hipMalloc(&kpos, 4ull * n));
+            BCHK(hipMalloc(&cnt, 4ull * ni));
+            BCHK(hipMalloc(&leaf_pos, 4ull * n));
+            BCHK(hipMalloc(&vals3, 4ull * n));
+            BCHK(hipMalloc(&pleaf3, 4ull * n));
+            BCHK(hipMalloc(&mctr, 4));
+            BCHK(hipMemsetAsync(mctr, 0, 4, st));
+            BCHK(rocprim::exclusive_scan(nullptr, scan_bytes, keep, kpos, 0u, n, rocprim::plus<uint32_t>(), st));
+            BCHK(hipMalloc(&scan_tmp, scan_bytes));
+            hipLaunchKernelGGL(k_ploc_init, dim3(G), dim3(B), 0, st, tris_unsorted, vals2, n, cl_node0, cl_box0);
+            BCHK(hipGetLastError());
+            uint32_t m = n;
+            while (m > 1) {
+                const uint32_t gm = (m + B - 1) / B;
+                hipLaunchKernelGGL(k_ploc_nn, dim3(gm), dim3(B), 0, st, cl_box0, m, nn);
+                hipLaunchKernelGGL(k_ploc_merge, dim3(gm), dim3(B), 0, st, cl_node0, cl_box0, m, nn, n, mctr, left,
+                                   right, pint, pleaf, cl_node1, cl_box1, keep);
+                BCHK(hipGetLastError());
+                BCHK(rocprim::exclusive_scan(scan_tmp, scan_bytes, keep, kpos, 0u, m, rocprim::plus<uint32_t>(), st));
+                hipLaunchKernelGGL(k_ploc_compact, dim3(gm), dim3(B), 0, st, cl_node1, cl_box1, m, keep, kpos,
+                                   cl_node0, cl_box0);
+                BCHK(hipGetLastError());
+                uint32_t tail[2];
+                BCHK(hipMemcpyAsync(&tail[0], kpos + (m - 1), 4, hipMemcpyDeviceToHost, st));
+                BCHK(hipMemcpyAsync(&tail[1], keep + (m - 1), 4, hipMemcpyDeviceToHost, st));
+                BCHK(hipStreamSynchronize(st));
+                const uint32_t m_next = tail[0] + tail[1];
+                if (m_next >= m) {
+                    rc = lt_fail(ctx, LT_ERR_HIP, "PLOC round merged no clusters");
+                    goto cleanup;
+                }
+                m = m_next;
+            }
+            // subtrees as contiguous runs: counts, ranges, new leaf order
+            BCHK(hipMemsetAsync(flags, 0, 4ull * ni, st));
+            hipLaunchKernelGGL(k_tree_count, dim3(G), dim3(B), 0, st, left, right, pint, pleaf, n, flags, cnt);
+            hipLaunchKernelGGL(k_tree_ranges, dim3(G), dim3(B), 0, st, left, right, pint, pleaf, cnt, n, rlo, rhi,
+                               leaf_pos);
+            hipLaunchKernelGGL(k_tree_relabel, dim3(G), dim3(B), 0, st, left, right, leaf_pos, vals2, pleaf, n, vals3,
+                               pleaf3);
+            BCHK(hipGetLastError());
+            std::swap(vals2, vals3);
+            std::swap(pleaf, pleaf3);
+            BCHK(hipMemsetAsync(flags, 0, 4ull * ni, st));
         }
         if (sah && n > 1) {
             BCHK(hipMalloc(&sah_R, sizeof(float) * 8 * (size_t)ni));
@@ -677,7 +909,8 @@ lt_status bvh_build(lt_context* ctx, const lt_build_opts* opts, lt_build_stats* 
 cleanup:
 #undef BCHK
     void* tmp[] = {tris_unsorted, cb, keys, vals, keys2, vals2, left, right, pint, pleaf, rlo, rhi, flags, ibox,
-                   lbox, q0, q1, ctr, sort_tmp, sah_R, sah_dec};
+                   lbox, q0, q1, ctr, sort_tmp, sah_R, sah_dec, cl_node0, cl_node1, cl_box0, cl_box1, nn, keep,
+                   kpos, cnt, leaf_pos, vals3, pleaf3, mctr, scan_tmp};
     for (void* p : tmp)
         if (p) (void)hipFree(p);
     if (rc != LT_OK) {

@@ -1,16 +1,19 @@
-"""BVH8 collapse quality on the GPU builder: the SAH-optimal collapse
-(LT_COLLAPSE_SAH, dynamic programming over the binary LBVH, Ylitie, Karras &
-Laine 2017) against the greedy largest-area expansion (LT_COLLAPSE_GREEDY)
-over the same binary tree.
+"""BVH8 quality on the GPU builder: the SAH-optimal collapse (LT_COLLAPSE_SAH,
+dynamic programming over the binary tree, Ylitie, Karras & Laine 2017)
+against the greedy largest-area expansion (LT_COLLAPSE_GREEDY) over the same
+binary LBVH, and the PLOC binary tree (LT_TREE_PLOC, the default: clustering
+by merged surface area, Meister & Bittner 2018) against the LBVH under the
+SAH collapse.
 
 The SAH cost of a BVH8, sum over nodes of A(node) * 1 plus sum over leaf
 slots of A(slot) * 0.5 * triangles (A = surface area), is computed here in
 numpy from the exported 128-B nodes (fp16 child planes dequantised).  The
 programme minimises exactly this cost over every collapse of the binary
 tree, and the greedy result is one such collapse, so the SAH build's cost is
-at most the greedy one's (up to the quantisation of the child boxes).  Both
-trees give the same hits (the traversal is exact either way); the visits per
-ray are printed for the record."""
+at most the greedy one's (up to the quantisation of the child boxes); PLOC's
+tree is expected to cost less than the LBVH's.  All trees give the same hits
+(the traversal is exact either way); the visits per ray are printed for the
+record."""
 import numpy as np
 import pytest
 
@@ -56,8 +59,9 @@ def test_sah_collapse_is_cheaper_than_greedy(gpu_ctx, kind, kw, max_leaf):
     tris = sc.triangles()
     rays = np.concatenate([random_rays(tris, 20000, 41), aimed_rays(tris, 20000, 42)])
     res = {}
-    for mode in ("greedy", "sah"):
-        st = gpu_ctx.build_accel(max_leaf, collapse=mode)
+    for mode, tree in (("greedy", "lbvh"), ("sah", "lbvh"), ("sah", "ploc")):
+        st = gpu_ctx.build_accel(max_leaf, collapse=mode, tree=tree)
+        mode = mode if tree == "lbvh" else "ploc"
         nodes, _ = gpu_ctx.export_bvh(st)
         hits = gpu_ctx.trace_closest(rays)
         nv, nt = gpu_ctx.trace_count(rays)
@@ -65,7 +69,10 @@ def test_sah_collapse_is_cheaper_than_greedy(gpu_ctx, kind, kw, max_leaf):
         print(f"[gpu] {kind} leaf={max_leaf} {mode}: sah cost {res[mode][0]:.6g}, {st.num_nodes} nodes, "
               f"{res[mode][2]:.2f} nodes and {res[mode][3]:.2f} triangles per ray", flush=True)
     assert res["sah"][0] <= res["greedy"][0] * (1 + 2e-3)
-    g, s = res["greedy"][4], res["sah"][4]
-    for f in ("inst", "prim"):
-        assert np.array_equal(g[f], s[f])
-    assert np.array_equal(g["t"].view(np.uint32), s["t"].view(np.uint32))
+    assert res["ploc"][0] <= res["sah"][0]
+    g = res["greedy"][4]
+    for other in ("sah", "ploc"):
+        s = res[other][4]
+        for f in ("inst", "prim"):
+            assert np.array_equal(g[f], s[f])
+        assert np.array_equal(g["t"].view(np.uint32), s["t"].view(np.uint32))
