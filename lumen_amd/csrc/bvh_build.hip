@@ -230,7 +230,10 @@ __device__ __forceinline__ float half_area(const Box& b) {
 // and a parent's index is below its children's), and the survivors are
 // compacted in order.  Ties are broken by the pair's indices, a strict order
 // under which the globally closest pair is mutual, so every round merges.
-constexpr int kPlocRadius = 16;
+#ifndef LT_PLOC_RADIUS
+#define LT_PLOC_RADIUS 16
+#endif
+constexpr int kPlocRadius = LT_PLOC_RADIUS;
 
 __device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
     Box r;
@@ -394,8 +397,11 @@ __global__ void k_tree_relabel(uint32_t* __restrict__ left, uint32_t* __restrict
 
 // SAH weights of a BVH8 node visit and of a triangle test (a watertight
 // test costs about half a node step)
+#ifndef LT_SAH_TRI_COST
+#define LT_SAH_TRI_COST 0.5f
+#endif
 constexpr float kSahNode = 1.0f;
-constexpr float kSahTri = 0.5f;
+constexpr float kSahTri = LT_SAH_TRI_COST;
 constexpr uint8_t kDecLeaf = 0xFF;
 
 // The dynamic programme at internal node p (children a, b):
