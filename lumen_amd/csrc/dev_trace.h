@@ -295,13 +295,19 @@ LT_DEV bool trace_step_dyn(const SceneView& sv, TraceState& s, const TraceStack&
             // at the first hit child, so that it names a hit child even if no
             // entry distance compares below +inf (an overflowed slab)
             int nearest = __builtin_ctz(hit_int);
-            float tbest = INFINITY;
+            // an occlusion query takes its children in slot order: any hit
+            // ends it, and without the nearest-first choice it also visits
+            // fewer nodes (a host replay of the BVH8s: 9.3 -> 9.0 nodes per
+            // shadow ray on a terrain, 9.5 -> 8.4 on the caustics scene)
+            if (!any) {
+                float tbest = INFINITY;
 #pragma unroll
-            for (int c = 0; c < 8; c++)
-                if (((hit_int >> c) & 1u) && tn_c[c] < tbest) {
-                    tbest = tn_c[c];
-                    nearest = c;
-                }
+                for (int c = 0; c < 8; c++)
+                    if (((hit_int >> c) & 1u) && tn_c[c] < tbest) {
+                        tbest = tn_c[c];
+                        nearest = c;
+                    }
+            }
             // the nearest is visited next; the others wait as one group and
             // are visited from the highest slot down (internal children fill
             // the low slots: child c is node child_base + c)

@@ -1,7 +1,8 @@
 // oracle_bvh8.cpp — host traversal of the device-built BVH8 (TEST INFRASTRUCTURE).
 //
 // Walks the exported 128-byte nodes with the same child decode, slab test and
-// nearest-child-first order as the device kernel, testing a node's leaf
+// child order as the device kernel (nearest child first for closest-hit
+// queries, slot order for occlusion queries), testing a node's leaf
 // triangles at once (the device queues them and tests one per step, so its
 // closest-hit bound shrinks later): its per-ray node and triangle counts are
 // the device's lower bound, and its hits must equal the device's and brute
@@ -122,13 +123,14 @@ extern "C" void ora_bvh8_trace(const void* nodes_v, uint32_t n_nodes, const void
                 }
             }
             if (done || !hit_int) continue;
-            int nearest = __builtin_ctz(hit_int);
+            int nearest = __builtin_ctz(hit_int);  // occlusion queries: slot order, as the device
             float tb = INFINITY;
-            for (int c = 0; c < 8; c++)
-                if (((hit_int >> c) & 1u) && tn[c] < tb) {
-                    tb = tn[c];
-                    nearest = c;
-                }
+            if (!any_hit)
+                for (int c = 0; c < 8; c++)
+                    if (((hit_int >> c) & 1u) && tn[c] < tb) {
+                        tb = tn[c];
+                        nearest = c;
+                    }
             for (int c = 0; c < 8; c++)
                 if (((hit_int >> c) & 1u) && c != nearest) stack.push_back(nd.child_base + nd.meta[c]);
             stack.push_back(nd.child_base + nd.meta[nearest]);
