@@ -1,6 +1,7 @@
 """Ground-truth validation: independent GPU estimates converge to the
 oracle's expectation (the time-to-RMSE ground truths are GPU renders, so they
-are checked here against the CPU oracle in distribution, not sample by sample).
+are checked here against the CPU oracle in distribution, not sample by sample):
+Path, BDPT and SPPM on the Cornell box, corrected VCM on the caustics scene.
 
 Both sides render independent 1-spp images (different seeds, so no replayed
 random stream is shared); per 8x8 block the difference of the two sample
@@ -37,7 +38,8 @@ def _accept(z) -> bool:
     return (np.abs(z) > 4.0).sum() <= 1 and np.mean(z ** 2) <= 2.0 and abs(z.sum()) / np.sqrt(len(z)) <= 4.0
 
 
-@pytest.mark.parametrize("integrator,kind", [("path", "cornell"), ("bdpt", "cornell"), ("vcm", "caustics")])
+@pytest.mark.parametrize("integrator,kind", [("path", "cornell"), ("bdpt", "cornell"), ("vcm", "caustics"),
+                                             ("sppm", "cornell")])
 def test_gpu_estimate_matches_oracle_in_distribution(gpu_ctx, integrator, kind):
     W, H = 64, 64
     sc = la.Scene.procedural(kind, seed=1)
@@ -56,6 +58,7 @@ def test_gpu_estimate_matches_oracle_in_distribution(gpu_ctx, integrator, kind):
         it.render(spp=1)
         g[k] = it.output_tex[..., :3].sum(-1)
     cam, depth, sky = it.camera, it.path_length, tuple(getattr(it, "sky_col", (0, 0, 0)))
+    base_radius = getattr(it, "base_radius", 0.03)
     it.destroy()
     acc = oracle.Accel(sc.desc)
     c = np.zeros((n_cpu, H, W), np.float64)
@@ -65,9 +68,11 @@ def test_gpu_estimate_matches_oracle_in_distribution(gpu_ctx, integrator, kind):
             o, _, _, _ = acc.render_path(cam, W, H, depth, sky_col=sky, frames=1, seed=s)
         elif integrator == "bdpt":
             o, _, _, _ = acc.render_bdpt(cam, W, H, depth, frames=1, seed=s)
-        else:
+        elif integrator == "vcm":
             o, _, _, _ = acc.render_vcm(cam, W, H, depth, frames=1, seed=s, radius_factor=2.0, use_vm=True,
                                         corrected=True, sky_col=sky)
+        else:  # the first frame of a fresh SPPM state on each side
+            o, _, _, _, _ = acc.render_sppm(cam, W, H, depth, frames=1, seed=s, base_radius=base_radius, sky_col=sky)
         c[k] = o[..., :3].sum(-1)
     z = _z_blocks(g, c)
     print(f"[gpu] {integrator} {kind}: {len(z)} blocks, max |z| {np.abs(z).max():.2f}, "
