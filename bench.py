@@ -270,7 +270,7 @@ def bench_adjoint(args) -> int:
     """C4: differentiable Cornell box (back wall = 16,384-triangle grid with
     256x256 albedo and roughness textures), 512x512, 32 spp, depth 6, 32
     texture variants per launch, L2 loss against a target render.  A step is
-    one optimisation iteration: forward (with the per-sample radiance cache),
+    one optimisation iteration: forward (writing the per-sample cache),
     loss gradient, backward."""
     import numpy as np
     import torch
@@ -332,9 +332,10 @@ def bench_adjoint(args) -> int:
     log(f"timed {steps} iterations in {wall:.3f}s: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, loss {loss:.4g}")
     # algorithmic bytes of one iteration from the counting variants of the
     # same kernels (untimed): BVH8 nodes x 128 B + triangles x 48 B, plus the
-    # radiance cache (12 B per sample, written forward and read backward) and
-    # the images / their adjoints (12 B per pixel and variant); texel reads and
-    # the gradient atomics are not counted
+    # sample cache, written forward and read backward (12 B of radiance and
+    # 4 B of flags per sample, a 16 B hit record per closest-hit ray of the
+    # forward), and the images / their adjoints (12 B per pixel and variant);
+    # texel reads and the gradient atomics are not counted
     adj.set_counting(True)
     img_c, cache_c = adj.forward(albs, roughs, seed=warmup + steps)
     cf = adj.counts()
@@ -342,6 +343,8 @@ def bench_adjoint(args) -> int:
     cb = adj.counts()
     adj.set_counting(False)
     cache_b = 12 * spp * V * W * H
+    if depth <= 10:  # the forward records its paths
+        cache_b += 4 * spp * V * W * H + 16 * cf["closest"]
     img_b = 12 * V * W * H
     fwd_bytes = cf["nodes"] * 128 + cf["tris"] * 48 + cache_b + img_b
     bwd_bytes = cb["nodes"] * 128 + cb["tris"] * 48 + cache_b + img_b
@@ -353,8 +356,10 @@ def bench_adjoint(args) -> int:
                 "algorithmic_bytes_per_launch": int(nbytes), "rays_per_launch": counts["closest"] + counts["shadow"]}
 
     roof = line("k_adjoint<true> (path replay with texel-gradient atomics)", bwd_bytes, bwd_ms, cb)
-    roof["other_kernels"] = {"k_adjoint<false>": line("k_adjoint<false> (forward, writes the radiance cache)",
+    roof["other_kernels"] = {"k_adjoint<false>": line("k_adjoint<false> (forward, writes the sample cache)",
                                                       fwd_bytes, fwd_ms, cf)}
+    roof["replay"] = ("the replay reads each path's hit records and traces only the occlusion rays the forward "
+                      "skipped; its time goes to the texel-gradient atomics and the BSDF derivatives")
     roof["traffic"] = traffic_from_profile(f"diff-cornell-wall16k-tex256-{W}x{H}-spp{spp}-v{V}-d{depth}", "adjoint")
     roof["note"] = ("the scene's BVH8 (16k-triangle wall and box) stays in L2 and the Infinity Cache, so the "
                     "algorithmic bytes can exceed the HBM peak; traffic is what reaches HBM")

@@ -536,13 +536,18 @@ lt_status lt_adjoint_destroy(lt_adjoint* adj);
 /* Texel grids of the two replaced textures (0 x 0 for a texture set to -1). */
 lt_status lt_adjoint_texture_size(lt_adjoint* adj, uint32_t* albedo_w, uint32_t* albedo_h, uint32_t* rough_w,
                                   uint32_t* rough_h);
-/* Floats of the optional per-sample radiance cache (spp * 3 * items). */
+/* Floats of the optional per-sample cache: spp * items * 3 (radiance), plus,
+ * for max_depth <= 10, spp * items * (8 * max_depth + 1) for the path records
+ * (each bounce's closest hit and NEE MIS hit, 16 B each, and a word of flags
+ * for the occlusion tests). */
 lt_status lt_adjoint_cache_floats(lt_adjoint* adj, size_t* n);
 /* All pointers are device pointers, float32, row-major:
  *   albedo [V][ah][aw][3], roughness [V][rh][rw], images [V][H][W][3].
  * sample_cache (may be NULL): filled by the forward, read by a backward with
  * the same camera, seed and texels, which then traces each path once instead
- * of twice.  stream: the hipStream_t the kernels run on, in order with the
+ * of twice (with path records: only the occlusion rays the forward skipped
+ * because their light's contribution was zero).  Another cache gives wrong
+ * gradients but no access out of bounds.  stream: the hipStream_t the kernels run on, in order with the
  * caller's other work on it (NULL: HIP's null stream, e.g. torch's default
  * stream); the calls are asynchronous. */
 lt_status lt_adjoint_forward(lt_adjoint* adj, const lt_camera* camera, uint32_t seed, const float* albedo,

@@ -140,8 +140,9 @@ def render_textures(adj: Adjoint, albedo, roughness, seed: int = 0, use_cache: b
     """Differentiable render of the texture variants: [V, H, W, 3].
 
     ``use_cache`` keeps each sample's radiance from the forward (12 B per
-    sample in HBM) so the backward replays every path once instead of
-    tracing it twice."""
+    sample in HBM) and, for path lengths up to 10, its path records (32 B
+    per bounce and 4 B of flags per sample), so the backward replays every
+    path from its records instead of tracing it again."""
     if albedo is None:
         albedo = torch.zeros(0, device="cuda")
     if roughness is None:

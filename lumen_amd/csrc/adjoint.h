@@ -23,7 +23,20 @@ struct AdjLaunch {
     uint32_t aw, ah, rw, rh;   // their texel grids
     uint32_t tiles_x;
     uint32_t n_items;          // tiles * 64 * variants
+    uint32_t rec_depth;        // bounces per sample in the cache's path records (0: none)
 };
+
+// Bounces per sample the sample cache records (3 flag bits per bounce in one
+// 32-bit word), and the cache's size in floats: per sample the radiance (3),
+// and with records two 16-byte hit records per bounce (the path's closest
+// hit and the NEE MIS ray's) and the flag word.
+constexpr uint32_t kAdjRecMaxDepth = 10;
+inline uint32_t adj_rec_depth(int32_t max_depth) {
+    return max_depth >= 1 && max_depth <= (int32_t)kAdjRecMaxDepth ? (uint32_t)max_depth : 0u;
+}
+inline size_t adj_cache_floats(uint32_t spp, uint32_t n_items, uint32_t rec_depth) {
+    return (size_t)spp * n_items * (3u + (rec_depth ? 8u * rec_depth + 1u : 0u));
+}
 
 // Device arrays of one call (all row-major, float32):
 //   albedo [V][ah][aw][3], rough [V][rh][rw]       variant texels

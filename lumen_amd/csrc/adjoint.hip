@@ -254,15 +254,50 @@ LT_DEV f3 safe_inv(f3 f) {
     return {f.x != 0.0f ? 1.0f / f.x : 0.0f, f.y != 0.0f ? 1.0f / f.y : 0.0f, f.z != 0.0f ? 1.0f / f.z : 0.0f};
 }
 
+// A hit as the sample cache records it: instance, primitive, barycentrics
+// (instance ~0: a miss), enough to rebuild the payload without a trace.  A
+// record naming no point of a triangle of the scene (a miss, or a cache that
+// no forward of this scene wrote) decodes as a miss, so any cache contents
+// stay in bounds and give finite hit points.
+LT_DEV void payload_from_record(const SceneView& sv, uint4 r, Payload& pl) {
+    const float u = __uint_as_float(r.z), v = __uint_as_float(r.w);
+    if (r.x >= sv.num_prims || r.y >= sv.tri_start[r.x + 1] - sv.tri_start[r.x] ||
+        !(u >= -1e-3f && v >= -1e-3f && u + v <= 1.001f))
+        pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
+    else
+        fill_payload(sv, r.x, r.y, u, v, pl);
+}
+
 template <bool COUNT>
-LT_DEV void closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl, Ctr& c) {
+LT_DEV void closest(const SceneView& sv, f3 o, f3 d, const TraceStack& stk, Payload& pl, Ctr& c,
+                    uint4* rec = nullptr) {
     HitRec h;
     if (COUNT) c.closest++;
-    if (trace_bvh<false, COUNT>(sv, o, d, ADJ_TMIN, ADJ_TMAX, h, stk, c.nodes, c.tris))
+    const bool hit = trace_bvh<false, COUNT>(sv, o, d, ADJ_TMIN, ADJ_TMAX, h, stk, c.nodes, c.tris);
+    const uint4 r = hit ? make_uint4(h.inst, h.prim, __float_as_uint(h.u), __float_as_uint(h.v))
+                        : make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+    if (rec) *rec = r;
+    if (hit)
         fill_payload(sv, h.inst, h.prim, h.u, h.v, pl);
     else
         pl.valid = false;  // ray.rmiss:13 — the other fields keep the last hit
 }
+
+// The path records of one sample in the sample cache: hit records
+// [k][bounce][2][item] (the path's closest hit, the NEE MIS ray's hit), a flag
+// word [k][item] with 3 bits per bounce (NEE occlusion traced, occluded, MIS
+// ray traced).  The forward writes them; the replay reads them instead of
+// tracing, and traces what the forward did not (a light whose contribution
+// was zero but whose derivative is not).
+struct AdjRec {
+    uint4* hits;  // nullptr: no records
+    uint32_t* flags;
+    size_t n;     // items
+    uint32_t depth, k, i;
+    LT_DEV uint4* hit(int bounce, int plane) const {
+        return hits + (((size_t)k * depth + (size_t)bounce) * 2u + (size_t)plane) * n + i;
+    }
+};
 
 template <bool COUNT>
 LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceStack& stk, Ctr& c) {
@@ -276,7 +311,9 @@ LT_DEV bool occluded(const SceneView& sv, f3 o, f3 d, float tmax, const TraceSta
 template <bool REPLAY, bool COUNT>
 LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, const float* rgh, uint32_t x,
                      uint32_t y, uint32_t word, f3 Lt, f3 delta, float* ga, float* gr, const TraceStack& stk,
-                     Ctr& cnt) {
+                     Ctr& cnt, const AdjRec& R) {
+    const bool rec_on = R.hits != nullptr;
+    uint32_t fl_word = (REPLAY && rec_on) ? R.flags[(size_t)R.k * R.n + R.i] : 0u;
     RNG seed = {x, y, word, 0u};
     const float r0 = rnd(seed);
     const float r1 = rnd(seed);
@@ -296,7 +333,10 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
     pl.pos = pl.n_s = pl.n_g = mk3(0.0f);
     const float pick = 1.0f / (float)sv.total_light_triangle_cnt;
     for (int depth = 0;; depth++) {
-        closest<COUNT>(sv, origin, direction, stk, pl, cnt);
+        if (REPLAY && rec_on)
+            payload_from_record(sv, *R.hit(depth, 0), pl);
+        else
+            closest<COUNT>(sv, origin, direction, stk, pl, cnt, rec_on ? R.hit(depth, 0) : nullptr);
         if (!pl.valid) {
             col += thr * ld3(L.sky_col);
             break;
@@ -339,7 +379,12 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                     const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bpdf / pw);
                     const f3 ka = mis * fabsf(cos_x) * Le / pw;
                     const f3 contrib = mis * fl.f * fabsf(cos_x) * Le / pw;
-                    if ((nz(contrib) || nz(fl.dfa) || nz(fl.dfr)) && !occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt)) {
+                    bool occ = false;
+                    if (nz(contrib) || nz(fl.dfa) || nz(fl.dfr)) {
+                        const uint32_t b = fl_word >> (3 * depth);
+                        occ = (rec_on && (b & 1u)) ? (b & 2u) != 0u : occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt);
+                    }
+                    if ((nz(contrib) || nz(fl.dfa) || nz(fl.dfr)) && !occ) {
                         res += contrib;
                         const f3 w = delta * thr / pick * ka;
                         qa += w * fl.dfa;
@@ -352,7 +397,11 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                 if (pw > 0.0f) {
                     const float mis = (rec.flags & LT_LIGHT_DELTA) ? 1.0f : 1.0f / (1.0f + bpdf / pw);
                     const f3 contrib = mis * f * fabsf(cos_x) * Le / pw;
-                    if (nz(contrib) && !occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt)) res += contrib;
+                    if (nz(contrib)) {
+                        const bool occ = occluded<COUNT>(sv, pp, wi, wi_len - EPS, stk, cnt);
+                        fl_word |= (occ ? 3u : 1u) << (3 * depth);
+                        if (!occ) res += contrib;
+                    }
                 }
             }
             if ((rec.flags & 7u) == LT_LIGHT_AREA) {
@@ -363,7 +412,12 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
                                      : DF{sample_bsdf(n_s, wo, dm.m, 1, side, wb, bp, cx, s0, s1, s2), mk3(0.0f),
                                           mk3(0.0f)};
                 if (bp != 0.0f) {
-                    closest<COUNT>(sv, pp, wb, stk, pl, cnt);
+                    if (REPLAY && rec_on && ((fl_word >> (3 * depth)) & 4u)) {
+                        payload_from_record(sv, *R.hit(depth, 1), pl);
+                    } else {
+                        closest<COUNT>(sv, pp, wb, stk, pl, cnt, (!REPLAY && rec_on) ? R.hit(depth, 1) : nullptr);
+                        if (!REPLAY) fl_word |= 4u << (3 * depth);
+                    }
                     if (pl.triangle_idx == rec.triangle_idx && pl.instance_idx == rec.instance_idx) {
                         const float wl = length(pl.pos - hp.pos);
                         const float gg = fabsf(dot(pl.n_s, -wb)) / (wl * wl);
@@ -397,6 +451,7 @@ LT_DEV f3 adj_sample(const SceneView& sv, const AdjLaunch& L, const float* alb, 
             thr *= f * fabsf(cos_t) / pdf;
         }
     }
+    if (!REPLAY && rec_on) R.flags[(size_t)R.k * R.n + R.i] = fl_word;
     return col;
 }
 
@@ -434,11 +489,19 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         const float* alb = L.aid >= 0 ? B.albedo + 3 * na * v : nullptr;
         const float* rgh = L.rid >= 0 ? B.rough + nr * v : nullptr;
         const size_t pix = npx * v + (size_t)y * L.width + x;
+        // the sample cache's path records of sample k of this item (none
+        // without a cache or beyond kAdjRecMaxDepth bounces)
+        auto rec_of = [&](uint32_t k) {
+            if (!B.cache || L.rec_depth == 0u) return AdjRec{nullptr, nullptr, 0, 0, 0, 0};
+            uint4* hits = reinterpret_cast<uint4*>(B.cache + (size_t)L.spp * 3u * L.n_items);
+            uint32_t* flags = reinterpret_cast<uint32_t*>(hits + (size_t)L.spp * L.rec_depth * 2u * L.n_items);
+            return AdjRec{hits, flags, (size_t)L.n_items, L.rec_depth, k, i};
+        };
         if (!BACKWARD) {
             f3 sum = mk3(0.0f);
             for (uint32_t k = 0; k < L.spp; k++) {
                 const f3 c = adj_sample<false, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f),
-                                                      nullptr, nullptr, stk, cnt);
+                                                      nullptr, nullptr, stk, cnt, rec_of(k));
                 if (B.cache) {
                     float* cp = B.cache + (size_t)k * 3 * L.n_items + i;
                     cp[0] = c.x;
@@ -462,10 +525,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                     c = mk3(cp[0], cp[L.n_items], cp[2 * (size_t)L.n_items]);
                 } else {
                     c = adj_sample<false, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, mk3(0.0f), mk3(0.0f), nullptr,
-                                                 nullptr, stk, cnt);
+                                                 nullptr, stk, cnt, AdjRec{nullptr, nullptr, 0, 0, 0, 0});
                 }
                 if (isnan(luminance(c))) continue;
-                adj_sample<true, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, c, delta, ga, gr, stk, cnt);
+                adj_sample<true, COUNT>(sv, L, alb, rgh, x, y, k ^ L.seed, c, delta, ga, gr, stk, cnt, rec_of(k));
             }
         }
     }

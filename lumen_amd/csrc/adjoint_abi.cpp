@@ -82,6 +82,7 @@ void fill_launch(const lt_adjoint* a, const lt_camera* cam, uint32_t seed, AdjLa
     L.rh = a->rh;
     L.tiles_x = a->tiles_x;
     L.n_items = a->n_items;
+    L.rec_depth = adj_rec_depth(a->cfg.max_depth);
 }
 
 // the scene may have been replaced since create: the texture grids the
@@ -162,7 +163,7 @@ lt_status lt_adjoint_texture_size(lt_adjoint* a, uint32_t* aw, uint32_t* ah, uin
 
 lt_status lt_adjoint_cache_floats(lt_adjoint* a, size_t* n) {
     if (!a || !n) return LT_ERR_INVALID_ARG;
-    *n = (size_t)a->cfg.spp * 3 * a->n_items;
+    *n = adj_cache_floats(a->cfg.spp, a->n_items, adj_rec_depth(a->cfg.max_depth));
     return LT_OK;
 }
 

@@ -40,6 +40,8 @@ SceneView make_scene_view(const lt_context* ctx) {
     v.lxf = s.lxf;
     v.nodes = ctx->bvh.nodes;
     v.tris = ctx->bvh.tris;
+    v.tri_start = s.tri_start;
+    v.num_prims = s.nprims;
     v.num_lights = s.nlights;
     v.num_nodes = ctx->bvh.built ? ctx->bvh.n_nodes : 0;
     v.refill_min = ctx->tuning.refill_min;

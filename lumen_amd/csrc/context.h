@@ -34,6 +34,8 @@ struct SceneView {
     const float* lxf;
     const ltb::Node8* nodes;
     const ltb::TriRec* tris;
+    const uint32_t* tri_start;  // num_prims + 1: first flattened triangle of each mesh
+    uint32_t num_prims;
     uint32_t num_lights;
     uint32_t num_nodes;
     uint32_t refill_min;  // wavefront trace stages (lt_tuning)

@@ -116,10 +116,11 @@ def test_adjoint_counting_variants(gpu_ctx):
     """The counting kernels (lt_adjoint_set_counting) give the same images bit
     for bit and the same gradients up to the order of their float atomics,
     and count the work: at least one camera ray per
-    sample, at least one node per ray; the cached replay traces exactly the
-    forward's closest-hit rays (every pixel has a non-zero adjoint) and at
-    least its occlusion rays (it also tests lights whose contribution is zero
-    but whose derivative is not)."""
+    sample, at least one node per ray; the cached replay reads the forward's
+    path records and traces no closest-hit ray, only the occlusion rays the
+    forward skipped (lights whose contribution is zero but whose derivative
+    is not); a replay without the cache traces every closest-hit ray twice
+    (the forward again, then the replay)."""
     print("[gpu] adjoint counting variants", flush=True)
     from lumen_amd.adjoint import Adjoint
     W, H, V, spp = 24, 16, 2, 2
@@ -135,16 +136,25 @@ def test_adjoint_counting_variants(gpu_ctx):
     cf = adj.counts()
     ga_c, gr_c = adj.backward(ta, tr, w, seed=3, cache=cache_c)
     cb = adj.counts()
+    adj.backward(ta, tr, w, seed=3, cache=None)
+    cn = adj.counts()
     adj.set_counting(False)
     torch.cuda.synchronize()
-    assert torch.equal(img, img_c) and torch.equal(cache, cache_c)
+    # the radiance and the flag words are written for every sample; hit
+    # records only for the bounces a path reaches
+    n_rad = spp * 3 * V * W * H
+    n_flags = spp * V * W * H
+    assert torch.equal(img, img_c) and torch.equal(cache[:n_rad], cache_c[:n_rad])
+    assert torch.equal(cache[-n_flags:], cache_c[-n_flags:])
     _cmp_grad(ga_c.cpu().numpy(), ga.cpu().numpy(), "albedo gradient, counting variant")
     _cmp_grad(gr_c.cpu().numpy(), gr.cpu().numpy(), "roughness gradient, counting variant")
     print(f"[gpu] adjoint counts forward {cf}, backward {cb}", flush=True)
     assert cf["closest"] >= V * W * H * spp and cf["nodes"] >= cf["closest"] + cf["shadow"]
     assert cf["tris"] > 0 and cf["shadow"] > 0 and cf["shadow_nodes"] <= cf["nodes"]
-    assert cb["closest"] == cf["closest"] and cb["tris"] - cb["shadow_tris"] == cf["tris"] - cf["shadow_tris"]
-    assert cb["nodes"] - cb["shadow_nodes"] == cf["nodes"] - cf["shadow_nodes"] and cb["shadow"] >= cf["shadow"]
+    assert cb["closest"] == 0 and cb["nodes"] == cb["shadow_nodes"] and cb["tris"] == cb["shadow_tris"]
+    assert cb["shadow"] < cf["shadow"]
+    assert cn["closest"] == 2 * cf["closest"] and cn["shadow"] >= cf["shadow"]
+    assert cn["nodes"] - cn["shadow_nodes"] == 2 * (cf["nodes"] - cf["shadow_nodes"])
     adj.close()
 
 
