@@ -471,6 +471,14 @@ LT_DEV f3 merge(const VcmLaunch& L, const Grid& G, int depth, f3 n_s, f3 wo, con
     return res;
 }
 
+// A cell's coordinates modulo 1024 in one word.  Within one query every cell
+// compared (those of its range, and that of a photon within the radius of
+// it) lies in the query's box of cells, a few cells wide, where this packing
+// is one to one.
+LT_DEV uint32_t cell_pack(int x, int y, int z) {
+    return (((uint32_t)x & 1023u) << 20) | (((uint32_t)y & 1023u) << 10) | ((uint32_t)z & 1023u);
+}
+
 // ---- merge pass (corrected estimator) ----
 // vcm_commons.glsl:251-296 summed over every photon in range, for the merge
 // queries the eye pass wrote.  A whole wave serves one query at a time: the
@@ -565,12 +573,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
         };
         for (int cbase = 0; cbase < ncells; cbase += 64) {
             const int ci = cbase + (int)lane;  // this lane's cell of the chunk: its photon range
-            uint32_t st = 0u, cnt = 0u;
-            int cx = 0, cy = 0, cz = 0;
+            uint32_t st = 0u, cnt = 0u, cpack = 0u;
             if (ci < ncells) {
-                cx = lo[0] + ci / (ny * nz);
-                cy = lo[1] + (ci / nz) % ny;
-                cz = lo[2] + ci % nz;
+                const int cx = lo[0] + ci / (ny * nz);
+                const int cy = lo[1] + (ci / nz) % ny;
+                const int cz = lo[2] + ci % nz;
+                cpack = cell_pack(cx, cy, cz);
                 const uint32_t h = cell_hash(cx, cy, cz, L.n_paths);
                 st = Gr.start[h];
                 cnt = Gr.start[h + 1] - st;
@@ -580,13 +588,15 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                 const uint32_t v = __shfl_up(incl, (unsigned)o);
                 if ((int)lane >= o) incl += v;
             }
+            // photon t of the chunk lies in lane j's cell at index base_j + t
+            // (modulo 2^32)
+            const uint32_t base = st - (incl - cnt);
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             if (COUNT) n_tested += total;
             // kMergeUnroll photons per lane per step: their positions are
             // loaded together, so one memory latency covers them all
             for (uint32_t t0 = 0; t0 < total; t0 += 64u * kMergeUnroll) {
-                uint32_t kk[kMergeUnroll];
-                int jx[kMergeUnroll], jy[kMergeUnroll], jz[kMergeUnroll];
+                uint32_t kk[kMergeUnroll], jc[kMergeUnroll];
                 bool live[kMergeUnroll];
                 float4 pa[kMergeUnroll];
 #pragma unroll
@@ -598,12 +608,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                         if (v <= t) j += step;
                     }
                     j = j < 63 ? j : 63;
-                    const uint32_t sj = __shfl(st, j), ej = __shfl(incl - cnt, j);
-                    jx[u] = __shfl(cx, j);
-                    jy[u] = __shfl(cy, j);
-                    jz[u] = __shfl(cz, j);
+                    const uint32_t bj = __shfl(base, j);
+                    jc[u] = __shfl(cpack, j);
                     live[u] = t < total;
-                    kk[u] = live[u] ? sj + (t - ej) : 0u;
+                    kk[u] = live[u] ? bj + t : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < kMergeUnroll; u++) pa[u] = live[u] ? Gr.ppos[kk[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -618,7 +626,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(LT_
                         if (dist_sqr <= r2) {
                             int g[3];
                             grid_idx_fast(L, ppos, g);
-                            keep = g[0] == jx[u] && g[1] == jy[u] && g[2] == jz[u];  // else another cell of the bucket
+                            keep = cell_pack(g[0], g[1], g[2]) == jc[u];  // else another cell of the bucket
                         }
                     }
                     const uint64_t m = __ballot(keep);
